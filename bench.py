@@ -1,0 +1,205 @@
+"""Benchmark: compound-eye rendering on one MI355X.
+
+Workload (the bee-scale plain-method configuration): two 5,500-ommatidium
+ellipsoidal-patch eyes (160 x 120 deg, acceptance U[1.5, 3] deg) at 128
+samples per ommatidium over a 2,189,058-triangle scene (1024^2 fBm heightfield
++ 2,000 16-segment cylinders), generated in-run from fixed seeds and loaded
+through loadGlTFscene.  One step renders both eyes once (2 eye-frames,
+1,408,000 rays) through renderFrame; inputs are resident in HBM before the
+timed region.  The metric is eye-frames/s; Grays/s, the trace kernel's
+roofline fraction and a CPU-oracle baseline are reported beside it.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import pathlib
+import platform
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = pathlib.Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+import __graft_entry__  # noqa: E402
+
+METRIC = "eye-frames/s at ≤1% RMS colour err; Grays/s; learned-adaptive vs uniform"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def cpu_baseline(cfg: dict, spp: int, pose12: np.ndarray, target_s: float = 10.0) -> dict:
+    """Times the CPU oracle on a bounded sample of the same workload (eye 0)."""
+    import oracle  # the checker, timed here as the CPU baseline
+
+    t0 = time.time()
+    osc = oracle.OracleScene.from_scene(cfg["scene"])
+    build_s = time.time() - t0
+    omm = cfg["eyes"][0]
+    threads = oracle.default_threads()
+    m = 64
+    t0 = time.perf_counter()
+    osc.render(omm[:m], pose12, spp=spp, threads=threads)
+    dt = max(time.perf_counter() - t0, 1e-3)
+    m = int(min(len(omm), max(64, m * target_s / dt)))
+    t0 = time.perf_counter()
+    osc.render(omm[:m], pose12, spp=spp, threads=threads)
+    dt = time.perf_counter() - t0
+    rays = m * spp
+    rays_per_s = rays / dt
+    eye_frame_rays = len(omm) * spp
+    cpu_model = platform.processor() or "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                cpu_model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {
+        "value": rays_per_s / eye_frame_rays,
+        "unit": "eye-frames/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"eye 0, first {m} of {len(omm)} ommatidia x {spp} spp ({rays} rays) in {dt:.2f} s; "
+                  f"{rays_per_s / 1e6:.3f} Mrays/s; oracle BVH build {build_s:.1f} s excluded; CPU {cpu_model}",
+        "grays_per_s": rays_per_s / 1e9,
+    }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this renderer runs on one GPU; --gpus above 1 is not supported "
+              "(the workload would shard by pose/eye, see the design notes)", file=sys.stderr)
+        return 2
+    cra = __graft_entry__.import_package()
+    from compound_ray_amd import eyerenderer as er
+    from compound_ray_amd import scenes
+
+    lib = cra.load_library()
+    lib.setVerbosity(False)
+    tmp = tempfile.mkdtemp(prefix="eyebench-")
+    t0 = time.time()
+    cfg = scenes.build_config(2, tmp)
+    log(f"generated C2 scene ({cfg['scene'].tri_count()} triangles) in {time.time() - t0:.1f} s")
+    spp = cfg["spp"]
+    t0 = time.time()
+    lib.loadGlTFscene(str(cfg["gltf"]).encode())
+    err = er.last_error(lib)
+    if err:
+        raise RuntimeError(err)
+    bvh = er.bvh_info(lib)
+    log(f"loaded + GPU LBVH in {time.time() - t0:.1f} s (build {bvh['build_ms']:.1f} ms, {bvh['nodes']} nodes)")
+    eyes = []
+    for i in range(lib.getCameraCount()):
+        lib.gotoCamera(i)
+        if lib.isCompoundEyeActive():
+            lib.setCurrentEyeSamplesPerOmmatidium(spp)
+            lib.setCameraPose(*[float(v) for v in cfg["pose"]])
+            eyes.append(i)
+    assert len(eyes) == 2
+    rays_per_step = 0
+    stats = {"nodes_visited": 0, "tris_tested": 0, "hits": 0}
+    omm_counts = []
+    for i in eyes:
+        lib.gotoCamera(i)
+        rays_per_step += int(lib.getCurrentEyeTotalSamples())
+        omm_counts.append(int(lib.getCurrentEyeOmmatidialCount()))
+        s = er.traversal_stats(lib)
+        for k in stats:
+            stats[k] += s[k]
+
+    def step():
+        for i in eyes:
+            lib.gotoCamera(i)
+            if lib.renderFrame() < 0:
+                raise RuntimeError(er.last_error(lib))
+
+    for _ in range(args.warmup):
+        step()
+    trace_ms = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        for i in eyes:
+            lib.gotoCamera(i)
+            if lib.renderFrame() < 0:
+                raise RuntimeError(er.last_error(lib))
+            trace_ms.append(er.timings_np(lib)[0])
+    elapsed = time.perf_counter() - t0
+    eye_frames = args.steps * len(eyes)
+    value = eye_frames / elapsed
+    grays = args.steps * rays_per_step / elapsed / 1e9
+    trace_avg_ms = float(np.mean(trace_ms))
+    # algorithmic bytes of one trace launch (one eye-frame): BVH nodes visited
+    # x 128 B + triangles tested x 48 B + shading fetch (3 x 16 B per hit) +
+    # ommatidium record (32 B / spp per ray) + per-sample record written (16 B)
+    rays_eye = rays_per_step / len(eyes)
+    bytes_launch = (stats["nodes_visited"] * 128 + stats["tris_tested"] * 48 + stats["hits"] * 48
+                    + rays_per_step * (32.0 / spp + 16.0)) / len(eyes)
+    achieved = bytes_launch / (trace_avg_ms * 1e-3) / 1e9
+    # RMS colour error of this render against a 4096-spp render of the same eye
+    lib.gotoCamera(eyes[0])
+    col128 = er.colours_np(lib)
+    lib.setCurrentEyeSamplesPerOmmatidium(4096)
+    lib.setCurrentEyeFrameIndex(1 << 20)
+    lib.renderFrame()
+    col4096 = er.colours_np(lib)
+    lib.setCurrentEyeSamplesPerOmmatidium(spp)
+    rms = float(np.sqrt(np.mean((col128.astype(np.float64) - col4096) ** 2)))
+    rms_rel = rms / float(np.sqrt(np.mean(col4096.astype(np.float64) ** 2)))
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "eye-frames/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: seeded C2 scene and eyes generated in-run, loaded via loadGlTFscene",
+        "config": {
+            "workload": "plain method, bee-scale: 2 eyes x 5,500 ommatidia x 128 spp, "
+                        f"{cfg['scene'].tri_count()} triangles (1024^2 fBm heightfield + 2,000 cylinders)",
+            "rays_per_step": rays_per_step,
+            "eye_frames_per_step": len(eyes),
+            "bvh": bvh,
+        },
+        "grays_per_s": grays,
+        "rms_colour_err_vs_4096spp": {"abs": rms, "rel": rms_rel, "note": "eye 0, GPU 4096-spp render as reference"},
+        "trace_kernel_ms": trace_avg_ms,
+        "traversal_per_ray": {"nodes": stats["nodes_visited"] / rays_per_step,
+                              "tris": stats["tris_tested"] / rays_per_step,
+                              "hit_fraction": stats["hits"] / rays_per_step},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_trace_samples", "bytes_per_launch": bytes_launch},
+    }
+    if not args.no_cpu_baseline:
+        lib.gotoCamera(eyes[0])
+        pose12 = er.pose_np(lib)
+        out["cpu_baseline"] = cpu_baseline(cfg, spp, pose12)
+    lib.stop()
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,911 @@
+// libEyeRenderer3 C ABI on MI355X (include/eye_renderer.h).
+//
+// Mirrors the global-singleton renderer of libEyeRenderer3/libEyeRenderer.cpp
+// (scene, camera list, current camera, render size, frame buffer), minus the
+// window: GPU state is created lazily on the first call that needs it, every
+// entry point catches its own failures, and nothing throws across the ABI.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <iostream>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/eye_renderer.h"
+#include "batch.h"
+#include "device_scene.h"
+#include "render.h"
+#include "scene_loader.h"
+
+using namespace eye;
+
+namespace {
+
+struct EyeState {
+  std::vector<Ommatidium> omms;
+  int spp = 1;
+  std::vector<uint32_t> counts;  // per-ommatidium sample counts, empty = uniform spp
+  std::vector<uint32_t> offsets; // counts' exclusive prefix sum (size omms + 1)
+  uint32_t frame = 0;
+  uint32_t eye_index = 0;
+  std::string data_path;
+  std::string shader = "single_dimension_fast";
+  Ommatidium* d_omms = nullptr;
+  size_t d_omms_cap = 0;
+  uint32_t* d_offsets = nullptr;
+  size_t d_offsets_cap = 0;
+  bool device_dirty = true;
+  float4* d_samples = nullptr;
+  size_t d_samples_cap = 0;
+  float4* d_colours = nullptr;
+  size_t d_colours_cap = 0;
+  bool has_render = false;
+  uint64_t rendered_rays = 0;
+
+  uint64_t total_samples() const {
+    if (counts.empty()) return (uint64_t)omms.size() * (uint64_t)spp;
+    return offsets.empty() ? 0 : offsets.back();
+  }
+  void restart_streams() { frame = 0; }
+};
+
+struct Cam {
+  std::string name;
+  CameraKind kind = kPerspective;
+  Pose pose;
+  int eye = -1;
+  float yfov_deg = 60.0f;
+};
+
+struct State {
+  bool verbose = true;
+  std::string last_error;
+  bool hip_ready = false;
+  hipStream_t stream = nullptr;
+  bool loaded = false;
+  HostScene host;
+  DeviceScene scene;
+  float* d_pos = nullptr;
+  float* d_col = nullptr;
+  uint32_t* d_mode = nullptr;
+  std::vector<Cam> cams;
+  std::vector<EyeState> eyes;
+  size_t current = 0;
+  int width = 400, height = 400;
+  std::vector<uint8_t> frame_host;
+  uchar4* d_frame = nullptr;
+  size_t d_frame_cap = 0;
+  bool frame_host_stale = false;
+  Pose* d_pose = nullptr;
+  uint32_t seed = 42;
+  int leaf_max = 4;
+  size_t tri_count = 0;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  double timings[4] = {0, 0, 0, 0};   // trace, reduce, projection, total GPU ms of the last render
+  unsigned long long* d_stats = nullptr;
+  BatchRenderer batch;
+  std::string name_buf;
+};
+
+State& st() {
+  static State s;
+  return s;
+}
+
+void set_error(const std::string& e) {
+  st().last_error = e;
+  std::cerr << "[PyEye] ERROR: " << e << std::endl;
+}
+
+void note(const std::string& msg) {
+  if (st().verbose) std::cout << "[PyEye] " << msg << std::endl;
+}
+
+// Throws std::runtime_error on HIP failure; entry points catch it.
+void hip_check(hipError_t e, const char* what) {
+  if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+void ensure_hip() {
+  State& s = st();
+  if (s.hip_ready) return;
+  int count = 0;
+  const hipError_t e = hipGetDeviceCount(&count);
+  if (e != hipSuccess || count == 0) throw std::runtime_error("no HIP device available");
+  hip_check(hipSetDevice(0), "hipSetDevice");
+  hip_check(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking), "hipStreamCreate");
+  hip_check(hipMalloc(&s.d_pose, sizeof(Pose)), "hipMalloc pose");
+  hip_check(hipMalloc(&s.d_stats, sizeof(unsigned long long) * 4), "hipMalloc stats");
+  for (hipEvent_t& e : s.ev) hip_check(hipEventCreate(&e), "hipEventCreate");
+  s.hip_ready = true;
+}
+
+template <typename T>
+void ensure_device(T** p, size_t* cap, size_t n) {
+  if (*cap >= n && *p) return;
+  if (*p) hip_check(hipFree(*p), "hipFree");
+  *p = nullptr;
+  *cap = 0;
+  if (n == 0) return;
+  hip_check(hipMalloc(p, sizeof(T) * n), "hipMalloc");
+  *cap = n;
+}
+
+Cam* current_cam() {
+  State& s = st();
+  if (s.cams.empty()) return nullptr;
+  return &s.cams[s.current];
+}
+
+EyeState* current_eye() {
+  Cam* c = current_cam();
+  if (!c || c->eye < 0) return nullptr;
+  return &st().eyes[(size_t)c->eye];
+}
+
+void free_scene() {
+  State& s = st();
+  free_device_scene(&s.scene);
+  if (s.d_pos) (void)hipFree(s.d_pos);
+  if (s.d_col) (void)hipFree(s.d_col);
+  if (s.d_mode) (void)hipFree(s.d_mode);
+  s.d_pos = s.d_col = nullptr;
+  s.d_mode = nullptr;
+  for (EyeState& e : s.eyes) {
+    if (e.d_omms) (void)hipFree(e.d_omms);
+    if (e.d_offsets) (void)hipFree(e.d_offsets);
+    if (e.d_samples) (void)hipFree(e.d_samples);
+    if (e.d_colours) (void)hipFree(e.d_colours);
+  }
+  s.eyes.clear();
+  s.cams.clear();
+  s.current = 0;
+  s.loaded = false;
+}
+
+void upload_eye(EyeState& e) {
+  State& s = st();
+  if (!e.device_dirty) return;
+  ensure_device(&e.d_omms, &e.d_omms_cap, e.omms.size());
+  if (!e.omms.empty())
+    hip_check(hipMemcpyAsync(e.d_omms, e.omms.data(), sizeof(Ommatidium) * e.omms.size(), hipMemcpyHostToDevice,
+                             s.stream),
+              "upload ommatidia");
+  if (!e.counts.empty()) {
+    ensure_device(&e.d_offsets, &e.d_offsets_cap, e.offsets.size());
+    hip_check(hipMemcpyAsync(e.d_offsets, e.offsets.data(), sizeof(uint32_t) * e.offsets.size(),
+                             hipMemcpyHostToDevice, s.stream),
+              "upload offsets");
+  }
+  hip_check(hipStreamSynchronize(s.stream), "sync");
+  e.device_dirty = false;
+}
+
+EyeLaunch launch_of(EyeState& e, const Pose* d_poses, int pose_count) {
+  EyeLaunch L;
+  L.omms = e.d_omms;
+  L.omm_count = (int)e.omms.size();
+  L.offsets = e.counts.empty() ? nullptr : e.d_offsets;
+  L.spp = e.spp;
+  L.rays_per_pose = e.total_samples();
+  L.poses = d_poses;
+  L.pose_count = pose_count;
+  L.pose_base = 0;
+  L.seed = st().seed;
+  L.eye = e.eye_index;
+  L.frame = e.frame;
+  return L;
+}
+
+void ensure_frame_buffer() {
+  State& s = st();
+  const size_t n = (size_t)std::max(s.width, 0) * (size_t)std::max(s.height, 0);
+  if (s.d_frame_cap < n || !s.d_frame) {
+    ensure_device(&s.d_frame, &s.d_frame_cap, n);
+    if (n) hip_check(hipMemsetAsync(s.d_frame, 0, sizeof(uchar4) * n, s.stream), "clear frame");
+  }
+}
+
+void build_scene_bvh() {
+  State& s = st();
+  std::string err;
+  hip_check(hipStreamSynchronize(s.stream), "sync");
+  if (!build_device_scene(s.d_pos, s.d_col, s.d_mode, (int)s.tri_count, s.leaf_max, s.stream, &s.scene,
+                          &err))
+    throw std::runtime_error(err);
+  s.scene.sky_mode = s.host.sky_mode;
+}
+
+double render_compound(EyeState& e, const Cam& c) {
+  State& s = st();
+  upload_eye(e);
+  const uint64_t total = e.total_samples();
+  ensure_device(&e.d_samples, &e.d_samples_cap, (size_t)total);
+  ensure_device(&e.d_colours, &e.d_colours_cap, e.omms.size());
+  ensure_frame_buffer();
+  hip_check(hipMemcpyAsync(s.d_pose, &c.pose, sizeof(Pose), hipMemcpyHostToDevice, s.stream), "upload pose");
+  EyeLaunch L = launch_of(e, s.d_pose, 1);
+  std::string err;
+  const auto t0 = std::chrono::steady_clock::now();
+  hip_check(hipEventRecord(s.ev[0], s.stream), "event");
+  if (!launch_trace_samples(s.scene, L, e.d_samples, s.stream, &err)) throw std::runtime_error(err);
+  hip_check(hipEventRecord(s.ev[1], s.stream), "event");
+  if (!launch_reduce(L, e.d_samples, e.d_colours, s.stream, &err)) throw std::runtime_error(err);
+  hip_check(hipEventRecord(s.ev[2], s.stream), "event");
+  bool ok = true;
+  if (e.shader == "single_dimension")
+    ok = launch_frame_single_dim(e.d_colours, L.omm_count, s.d_frame, s.width, s.height, s.stream, &err);
+  else if (e.shader == "raw_ommatidial_samples")
+    ok = e.counts.empty() ? launch_frame_raw_samples(e.d_samples, L.omm_count, e.spp, s.d_frame, s.width, s.height,
+                                                     s.stream, &err)
+                          : true;
+  else
+    ok = launch_frame_single_dim_fast(e.d_colours, L.omm_count, s.d_frame, s.width, s.stream, &err);
+  if (!ok) throw std::runtime_error(err);
+  hip_check(hipEventRecord(s.ev[3], s.stream), "event");
+  hip_check(hipStreamSynchronize(s.stream), "render");
+  const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  float f[3] = {0, 0, 0};
+  for (int k = 0; k < 3; ++k) (void)hipEventElapsedTime(&f[k], s.ev[k], s.ev[k + 1]);
+  s.timings[0] = f[0];
+  s.timings[1] = f[1];
+  s.timings[2] = f[2];
+  s.timings[3] = (double)f[0] + f[1] + f[2];
+  e.frame += 1;
+  e.has_render = true;
+  e.rendered_rays = total;
+  s.frame_host_stale = true;
+  return ms;
+}
+
+void recompute_offsets(EyeState& e) {
+  e.offsets.clear();
+  if (e.counts.empty()) return;
+  e.offsets.resize(e.counts.size() + 1);
+  uint64_t run = 0;
+  for (size_t i = 0; i < e.counts.size(); ++i) {
+    e.offsets[i] = (uint32_t)run;
+    run += e.counts[i];
+  }
+  if (run > 0xffffffffull) throw std::runtime_error("more than 2^32 samples per eye");
+  e.offsets.back() = (uint32_t)run;
+}
+
+void sync_frame_host() {
+  State& s = st();
+  const size_t n = (size_t)std::max(s.width, 0) * (size_t)std::max(s.height, 0);
+  if (s.frame_host.size() != n * 4) s.frame_host.assign(n * 4, 0);
+  if (!s.frame_host_stale || !s.hip_ready || !s.d_frame || n == 0) return;
+  hip_check(hipMemcpy(s.frame_host.data(), s.d_frame, n * 4, hipMemcpyDeviceToHost), "frame readback");
+  s.frame_host_stale = false;
+}
+
+#define ABI_GUARD_BEGIN try {
+#define ABI_GUARD_END(ret)                      \
+  }                                             \
+  catch (const std::exception& ex) {            \
+    set_error(ex.what());                       \
+    ret;                                        \
+  }                                             \
+  catch (...) {                                 \
+    set_error("unknown error");                 \
+    ret;                                        \
+  }
+
+}  // namespace
+
+extern "C" {
+
+void setVerbosity(bool v) { st().verbose = v; }
+
+void loadGlTFscene(const char* filepath) {
+  ABI_GUARD_BEGIN
+  State& s = st();
+  if (!filepath) throw std::runtime_error("loadGlTFscene: null path");
+  HostScene hs;
+  std::string err;
+  if (!load_gltf(filepath, &hs, &err)) throw std::runtime_error(err);
+  for (const std::string& w : hs.warnings) std::cerr << "[PyEye] glTF warning: " << w << std::endl;
+  ensure_hip();
+  free_scene();
+  s.host = std::move(hs);
+  const size_t n = s.host.tri_count();
+  s.tri_count = n;
+  if (n) {
+    hip_check(hipMalloc(&s.d_pos, sizeof(float) * 9 * n), "hipMalloc positions");
+    hip_check(hipMalloc(&s.d_col, sizeof(float) * 9 * n), "hipMalloc colours");
+    hip_check(hipMalloc(&s.d_mode, sizeof(uint32_t) * n), "hipMalloc modes");
+    hip_check(hipMemcpy(s.d_pos, s.host.pos.data(), sizeof(float) * 9 * n, hipMemcpyHostToDevice), "upload");
+    hip_check(hipMemcpy(s.d_col, s.host.col.data(), sizeof(float) * 9 * n, hipMemcpyHostToDevice), "upload");
+    hip_check(hipMemcpy(s.d_mode, s.host.mode.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice), "upload");
+  }
+  build_scene_bvh();
+  // release the host copies of the geometry; the device keeps them for rebuilds
+  s.host.pos.clear(); s.host.pos.shrink_to_fit();
+  s.host.col.clear(); s.host.col.shrink_to_fit();
+  s.host.mode.clear(); s.host.mode.shrink_to_fit();
+  uint32_t eye_idx = 0;
+  for (CameraDesc& cd : s.host.cameras) {
+    Cam c;
+    c.name = cd.name;
+    c.kind = cd.kind;
+    c.pose = cd.pose;
+    c.yfov_deg = cd.yfov_deg;
+    if (cd.kind == kCompound) {
+      EyeState e;
+      e.omms = std::move(cd.ommatidia);
+      e.eye_index = eye_idx++;
+      e.data_path = cd.eye_path;
+      e.shader = cd.projection;
+      c.eye = (int)s.eyes.size();
+      s.eyes.push_back(std::move(e));
+    }
+    s.cams.push_back(std::move(c));
+  }
+  s.host.cameras.clear();
+  if (s.cams.empty()) {  // MulticamScene::getCamera adds a default perspective camera
+    Cam c;
+    c.name = "Default Camera";
+    pose_reset(&c.pose);
+    s.cams.push_back(c);
+  }
+  s.current = 0;
+  s.loaded = true;
+  s.last_error.clear();
+  note("Loaded scene '" + std::string(filepath) + "': " + std::to_string(n) + " triangles, " +
+       std::to_string(s.cams.size()) + " cameras, " + std::to_string(s.scene.node_count) + " BVH nodes.");
+  ABI_GUARD_END(return)
+}
+
+void stop(void) {
+  ABI_GUARD_BEGIN
+  note("Cleaning eye renderer resources.");
+  State& s = st();
+  if (s.hip_ready) {
+    (void)hipStreamSynchronize(s.stream);
+    free_scene();
+    s.batch.release();
+    if (s.d_frame) (void)hipFree(s.d_frame);
+    s.d_frame = nullptr;
+    s.d_frame_cap = 0;
+  }
+  ABI_GUARD_END(return)
+}
+
+void setRenderSize(int w, int h) {
+  ABI_GUARD_BEGIN
+  State& s = st();
+  s.width = std::max(w, 0);
+  s.height = std::max(h, 0);
+  note("Resizing rendering buffer to (" + std::to_string(w) + ", " + std::to_string(h) + ").");
+  s.frame_host.assign((size_t)s.width * (size_t)s.height * 4, 0);
+  if (s.hip_ready) {
+    if (s.d_frame) hip_check(hipFree(s.d_frame), "hipFree");
+    s.d_frame = nullptr;
+    s.d_frame_cap = 0;
+    ensure_frame_buffer();
+  }
+  s.frame_host_stale = false;
+  ABI_GUARD_END(return)
+}
+
+double renderFrame(void) {
+  ABI_GUARD_BEGIN
+  State& s = st();
+  if (!s.loaded) throw std::runtime_error("renderFrame: no scene loaded");
+  Cam* c = current_cam();
+  double ms = 0.0;
+  if (c && c->eye >= 0) {
+    ms = render_compound(s.eyes[(size_t)c->eye], *c);
+  } else {
+    throw std::runtime_error("renderFrame: regular (non-compound) cameras are not rendered by this library");
+  }
+  s.last_error.clear();
+  if (s.verbose) std::cout << "[PyEye] Rendered frame in " << ms << "ms." << std::endl;
+  return ms;
+  ABI_GUARD_END(return -1.0)
+}
+
+void displayFrame(void) {}
+
+void saveFrameAs(char* ppmFilename) {
+  ABI_GUARD_BEGIN
+  State& s = st();
+  if (!ppmFilename) throw std::runtime_error("saveFrameAs: null file name");
+  sync_frame_host();
+  FILE* f = fopen(ppmFilename, "wb");
+  if (!f) throw std::runtime_error(std::string("saveFrameAs: cannot open '") + ppmFilename + "'");
+  fprintf(f, "P6\n%d %d\n255\n", s.width, s.height);
+  // rows are written bottom-up (the frame is stored upside down, sutil.cpp:82-103)
+  std::vector<uint8_t> row((size_t)s.width * 3);
+  for (int y = s.height - 1; y >= 0; --y) {
+    for (int x = 0; x < s.width; ++x)
+      for (int k = 0; k < 3; ++k) row[(size_t)x * 3 + k] = s.frame_host[((size_t)y * s.width + x) * 4 + k];
+    fwrite(row.data(), 1, row.size(), f);
+  }
+  fclose(f);
+  note(std::string("Saved render as '") + ppmFilename + "'");
+  ABI_GUARD_END(return)
+}
+
+unsigned char* getFramePointer(void) {
+  ABI_GUARD_BEGIN
+  note("Retrieving frame pointer...");
+  sync_frame_host();
+  State& s = st();
+  return s.frame_host.empty() ? nullptr : s.frame_host.data();
+  ABI_GUARD_END(return nullptr)
+}
+
+size_t getCameraCount(void) { return st().cams.size(); }
+
+void nextCamera(void) { gotoCamera((int)st().current + 1); }
+void previousCamera(void) { gotoCamera((int)st().current - 1); }
+size_t getCurrentCameraIndex(void) { return st().current; }
+
+const char* getCurrentCameraName(void) {
+  Cam* c = current_cam();
+  return c ? c->name.c_str() : "";
+}
+
+void gotoCamera(int index) {
+  State& s = st();
+  const int n = (int)s.cams.size();
+  if (n == 0) return;
+  s.current = (size_t)(((index % n) + n) % n);
+}
+
+bool gotoCameraByName(char* name) {
+  State& s = st();
+  if (!name) return false;
+  for (size_t i = 0; i < s.cams.size(); ++i)
+    if (s.cams[i].name == name) {
+      s.current = i;
+      return true;
+    }
+  // the reference leaves the camera index past the last camera, i.e. at 0
+  if (!s.cams.empty()) s.current = 0;
+  return false;
+}
+
+void setCameraPosition(float x, float y, float z) {
+  Cam* c = current_cam();
+  if (!c) return;
+  c->pose.pos[0] = x; c->pose.pos[1] = y; c->pose.pos[2] = z;
+}
+
+void getCameraPosition(float* x, float* y, float* z) {
+  Cam* c = current_cam();
+  const float p[3] = {c ? c->pose.pos[0] : 0.0f, c ? c->pose.pos[1] : 0.0f, c ? c->pose.pos[2] : 0.0f};
+  if (x) *x = p[0];
+  if (y) *y = p[1];
+  if (z) *z = p[2];
+}
+
+void setCameraLocalSpace(float lxx, float lxy, float lxz, float lyx, float lyy, float lyz, float lzx, float lzy,
+                         float lzz) {
+  Cam* c = current_cam();
+  if (!c) return;
+  c->pose.x[0] = lxx; c->pose.x[1] = lxy; c->pose.x[2] = lxz;
+  c->pose.y[0] = lyx; c->pose.y[1] = lyy; c->pose.y[2] = lyz;
+  c->pose.z[0] = lzx; c->pose.z[1] = lzy; c->pose.z[2] = lzz;
+}
+
+void rotateCameraAround(float angle, float x, float y, float z) {
+  Cam* c = current_cam();
+  if (c) pose_rotate_around(&c->pose, angle, x, y, z);
+}
+
+void rotateCameraLocallyAround(float angle, float x, float y, float z) {
+  Cam* c = current_cam();
+  if (!c) return;
+  // transformToLocal, DataRecordCamera.h:57-65
+  const Pose& p = c->pose;
+  const V3 a = (v3(p.x[0], p.x[1], p.x[2]) * x + v3(p.y[0], p.y[1], p.y[2]) * y) + v3(p.z[0], p.z[1], p.z[2]) * z;
+  pose_rotate_around(&c->pose, angle, a.x, a.y, a.z);
+}
+
+void translateCamera(float x, float y, float z) {
+  Cam* c = current_cam();
+  if (c) pose_move(&c->pose, x, y, z);
+}
+
+void translateCameraLocally(float x, float y, float z) {
+  Cam* c = current_cam();
+  if (!c) return;
+  const Pose& p = c->pose;
+  const V3 a = (v3(p.x[0], p.x[1], p.x[2]) * x + v3(p.y[0], p.y[1], p.y[2]) * y) + v3(p.z[0], p.z[1], p.z[2]) * z;
+  pose_move(&c->pose, a.x, a.y, a.z);
+}
+
+void resetCameraPose(void) {
+  Cam* c = current_cam();
+  if (c) pose_reset(&c->pose);
+}
+
+void setCameraPose(float posX, float posY, float posZ, float rotX, float rotY, float rotZ) {
+  Cam* c = current_cam();
+  if (!c) return;
+  pose_reset(&c->pose);
+  pose_rotate_around(&c->pose, rotX, 1.0f, 0.0f, 0.0f);
+  pose_rotate_around(&c->pose, rotY, 0.0f, 1.0f, 0.0f);
+  pose_rotate_around(&c->pose, rotZ, 0.0f, 0.0f, 1.0f);
+  pose_move(&c->pose, posX, posY, posZ);
+}
+
+bool isCompoundEyeActive(void) { return current_eye() != nullptr; }
+
+void setCurrentEyeSamplesPerOmmatidium(int s) {
+  EyeState* e = current_eye();
+  if (!e) return;
+  e->spp = std::max(1, s);
+  e->restart_streams();
+}
+
+int getCurrentEyeSamplesPerOmmatidium(void) {
+  EyeState* e = current_eye();
+  return e ? e->spp : -1;
+}
+
+void changeCurrentEyeSamplesPerOmmatidiumBy(int s) {
+  EyeState* e = current_eye();
+  if (!e) return;
+  setCurrentEyeSamplesPerOmmatidium(e->spp + s);
+}
+
+size_t getCurrentEyeOmmatidialCount(void) {
+  EyeState* e = current_eye();
+  return e ? e->omms.size() : 0;
+}
+
+void setOmmatidia(OmmatidiumPacket* omms, size_t count) {
+  ABI_GUARD_BEGIN
+  EyeState* e = current_eye();
+  if (!e) return;
+  if (count && !omms) throw std::runtime_error("setOmmatidia: null array");
+  static_assert(sizeof(OmmatidiumPacket) == sizeof(Ommatidium), "packet layout");
+  if (count != e->omms.size()) {
+    e->restart_streams();  // CompoundEye::reconfigureOmmatidialCount reallocates the random states
+    e->counts.clear();
+    e->offsets.clear();
+  }
+  e->omms.resize(count);
+  if (count) memcpy(e->omms.data(), omms, sizeof(Ommatidium) * count);
+  e->device_dirty = true;
+  ABI_GUARD_END(return)
+}
+
+const char* getCurrentEyeDataPath(void) {
+  EyeState* e = current_eye();
+  return e ? e->data_path.c_str() : "";
+}
+
+void setCurrentEyeShaderName(char* name) {
+  EyeState* e = current_eye();
+  if (!e || !name) return;
+  e->shader = name;
+}
+
+bool isInsideHitGeometry(float x, float y, float z, char* name) {
+  ABI_GUARD_BEGIN
+  State& s = st();
+  if (!name) return false;
+  const HitboxMesh* hb = nullptr;
+  for (const HitboxMesh& h : s.host.hitboxes)
+    if (h.name == name) { hb = &h; break; }
+  if (!hb) {
+    std::cerr << "WARNING: No hitbox with the given name \"" << name << "\" is present in the scene." << std::endl;
+    return false;
+  }
+  // Parity of crossings of a +x ray from outside the mesh bounds to the point
+  // (sutil/hitscanprocessing.cpp:20-82), evaluated in world space.
+  const V3 p = v3(x, y, z);
+  const V3 start = v3(hb->lo[0] - 1.0f, y, z);
+  const V3 dir = normalize(p - start);
+  unsigned crossings = 0;
+  for (size_t t = 0; t + 9 <= hb->tris.size(); t += 9) {
+    const V3 p0 = v3(hb->tris[t], hb->tris[t + 1], hb->tris[t + 2]);
+    const V3 p1 = v3(hb->tris[t + 3], hb->tris[t + 4], hb->tris[t + 5]);
+    const V3 p2 = v3(hb->tris[t + 6], hb->tris[t + 7], hb->tris[t + 8]);
+    const V3 nrm = normalize(cross(p1 - p0, p2 - p0));
+    const float den = dot(nrm, dir);
+    if (den == 0.0f) continue;
+    const float dist = dot(p0 - start, nrm) / den;
+    if (dist == 0.0f) continue;
+    const V3 hit = start + dir * dist;
+    if (dist < 0.0f || hit.x > p.x) continue;
+    if (dot(nrm, cross(p1 - p0, hit - p0)) < 0.0f) continue;
+    if (dot(nrm, cross(p2 - p1, hit - p1)) < 0.0f) continue;
+    if (dot(nrm, cross(p0 - p2, hit - p2)) < 0.0f) continue;
+    ++crossings;
+  }
+  return (crossings % 2u) == 1u;
+  ABI_GUARD_END(return false)
+}
+
+static EyeFloat3 bounds_of(const char* name, bool max_side) {
+  State& s = st();
+  EyeFloat3 r = {0.0f, 0.0f, 0.0f};
+  if (!name) return r;
+  for (const HitboxMesh& h : s.host.hitboxes)
+    if (h.name == name) {
+      const float* b = max_side ? h.hi : h.lo;
+      r.x = b[0]; r.y = b[1]; r.z = b[2];
+      return r;
+    }
+  for (size_t m = 0; m < s.host.mesh_names.size(); ++m)
+    if (s.host.mesh_names[m] == name && m < s.host.mesh_bounds.size()) {
+      const float* b = s.host.mesh_bounds[m].data() + (max_side ? 3 : 0);
+      r.x = b[0]; r.y = b[1]; r.z = b[2];
+      return r;
+    }
+  return r;
+}
+
+EyeFloat3 getGeometryMaxBounds(char* name) { return bounds_of(name, true); }
+EyeFloat3 getGeometryMinBounds(char* name) { return bounds_of(name, false); }
+
+void* getWindowPointer(void) { return nullptr; }
+
+const char* eyeRendererGetLastError(void) { return st().last_error.c_str(); }
+
+size_t getOmmatidialColours(float* rgb, size_t count) {
+  ABI_GUARD_BEGIN
+  EyeState* e = current_eye();
+  if (!e || !e->has_render || !rgb) return 0;
+  const size_t n = std::min(count, e->omms.size());
+  std::vector<float4> tmp(n);
+  if (n) hip_check(hipMemcpy(tmp.data(), e->d_colours, sizeof(float4) * n, hipMemcpyDeviceToHost), "readback");
+  for (size_t i = 0; i < n; ++i) {
+    rgb[3 * i] = tmp[i].x;
+    rgb[3 * i + 1] = tmp[i].y;
+    rgb[3 * i + 2] = tmp[i].z;
+  }
+  return n;
+  ABI_GUARD_END(return 0)
+}
+
+void setCurrentEyeSampleCounts(const uint32_t* counts, size_t count) {
+  ABI_GUARD_BEGIN
+  EyeState* e = current_eye();
+  if (!e) return;
+  if (count == 0) {
+    e->counts.clear();
+    e->offsets.clear();
+  } else {
+    if (count != e->omms.size())
+      throw std::runtime_error("setCurrentEyeSampleCounts: count must equal the ommatidial count");
+    if (!counts) throw std::runtime_error("setCurrentEyeSampleCounts: null array");
+    e->counts.assign(counts, counts + count);
+    recompute_offsets(*e);
+  }
+  e->restart_streams();
+  e->device_dirty = true;
+  ABI_GUARD_END(return)
+}
+
+uint64_t getCurrentEyeTotalSamples(void) {
+  EyeState* e = current_eye();
+  return e ? e->total_samples() : 0;
+}
+
+void setRandomSeed(uint32_t seed) { st().seed = seed; }
+
+uint32_t getCurrentEyeFrameIndex(void) {
+  EyeState* e = current_eye();
+  return e ? e->frame : 0;
+}
+
+void setCurrentEyeFrameIndex(uint32_t frame) {
+  EyeState* e = current_eye();
+  if (e) e->frame = frame;
+}
+
+uint32_t getCurrentEyeIndex(void) {
+  EyeState* e = current_eye();
+  return e ? e->eye_index : 0xffffffffu;
+}
+
+int getLastRenderTimings(double* out4) {
+  if (!out4) return -1;
+  for (int k = 0; k < 4; ++k) out4[k] = st().timings[k];
+  return 0;
+}
+
+int getCurrentEyeTraversalStats(uint64_t* out3) {
+  ABI_GUARD_BEGIN
+  State& s = st();
+  EyeState* e = current_eye();
+  Cam* c = current_cam();
+  if (!e || !out3 || !s.loaded) return -1;
+  upload_eye(*e);
+  hip_check(hipMemsetAsync(s.d_stats, 0, sizeof(unsigned long long) * 4, s.stream), "clear stats");
+  hip_check(hipMemcpyAsync(s.d_pose, &c->pose, sizeof(Pose), hipMemcpyHostToDevice, s.stream), "upload pose");
+  EyeLaunch L = launch_of(*e, s.d_pose, 1);
+  std::string err;
+  if (!launch_trace_stats(s.scene, L, s.d_stats, s.stream, &err)) throw std::runtime_error(err);
+  unsigned long long h[4];
+  hip_check(hipMemcpyAsync(h, s.d_stats, sizeof(h), hipMemcpyDeviceToHost, s.stream), "stats readback");
+  hip_check(hipStreamSynchronize(s.stream), "stats");
+  out3[0] = h[0];
+  out3[1] = h[1];
+  out3[2] = h[2];
+  return 0;
+  ABI_GUARD_END(return -1)
+}
+
+int getCameraPose(float* out12) {
+  Cam* c = current_cam();
+  if (!c || !out12) return -1;
+  static_assert(sizeof(Pose) == 12 * sizeof(float), "pose layout");
+  memcpy(out12, &c->pose, sizeof(Pose));
+  return 0;
+}
+
+size_t getCurrentEyeRays(float* rays, size_t count) {
+  ABI_GUARD_BEGIN
+  State& s = st();
+  EyeState* e = current_eye();
+  Cam* c = current_cam();
+  if (!e || !rays || !s.loaded) return 0;
+  upload_eye(*e);
+  const uint64_t total = e->total_samples();
+  const size_t n = (size_t)std::min<uint64_t>(total, count);
+  if (n == 0) return 0;
+  Ray* d_rays = nullptr;
+  hip_check(hipMalloc(&d_rays, sizeof(Ray) * total), "hipMalloc rays");
+  hip_check(hipMemcpyAsync(s.d_pose, &c->pose, sizeof(Pose), hipMemcpyHostToDevice, s.stream), "upload pose");
+  EyeLaunch L = launch_of(*e, s.d_pose, 1);
+  std::string err;
+  const bool ok = launch_generate_rays(L, d_rays, s.stream, &err);
+  hipError_t he = hipStreamSynchronize(s.stream);
+  if (ok && he == hipSuccess) he = hipMemcpy(rays, d_rays, sizeof(Ray) * n, hipMemcpyDeviceToHost);
+  (void)hipFree(d_rays);
+  if (!ok) throw std::runtime_error(err);
+  hip_check(he, "ray readback");
+  return n;
+  ABI_GUARD_END(return 0)
+}
+
+size_t getCurrentEyeSamples(float* samples, size_t count) {
+  ABI_GUARD_BEGIN
+  EyeState* e = current_eye();
+  if (!e || !e->has_render || !samples) return 0;
+  const size_t n = (size_t)std::min<uint64_t>(e->rendered_rays, count);
+  if (n) hip_check(hipMemcpy(samples, e->d_samples, sizeof(float4) * n, hipMemcpyDeviceToHost), "readback");
+  return n;
+  ABI_GUARD_END(return 0)
+}
+
+int traceRays(const float* rays, size_t count, int32_t* primIds, float* tbb) {
+  ABI_GUARD_BEGIN
+  State& s = st();
+  if (!s.loaded) throw std::runtime_error("traceRays: no scene loaded");
+  if (count == 0) return 0;
+  if (!rays || !primIds || !tbb) throw std::runtime_error("traceRays: null argument");
+  Ray* d_rays = nullptr;
+  int32_t* d_hit = nullptr;
+  float4* d_tbb = nullptr;
+  std::string err;
+  bool ok = false;
+  hipError_t he = hipMalloc(&d_rays, sizeof(Ray) * count);
+  if (he == hipSuccess) he = hipMalloc(&d_hit, sizeof(int32_t) * count);
+  if (he == hipSuccess) he = hipMalloc(&d_tbb, sizeof(float4) * count);
+  if (he == hipSuccess) he = hipMemcpy(d_rays, rays, sizeof(Ray) * count, hipMemcpyHostToDevice);
+  if (he == hipSuccess) {
+    ok = launch_trace_rays(s.scene, d_rays, count, d_hit, d_tbb, s.stream, &err);
+    he = hipStreamSynchronize(s.stream);
+  }
+  std::vector<float4> t4;
+  if (ok && he == hipSuccess) {
+    he = hipMemcpy(primIds, d_hit, sizeof(int32_t) * count, hipMemcpyDeviceToHost);
+    t4.resize(count);
+    if (he == hipSuccess) he = hipMemcpy(t4.data(), d_tbb, sizeof(float4) * count, hipMemcpyDeviceToHost);
+  }
+  (void)hipFree(d_rays);
+  (void)hipFree(d_hit);
+  (void)hipFree(d_tbb);
+  if (!ok && !err.empty()) throw std::runtime_error(err);
+  hip_check(he, "traceRays");
+  for (size_t i = 0; i < count; ++i) {
+    tbb[3 * i] = t4[i].x;
+    tbb[3 * i + 1] = t4[i].y;
+    tbb[3 * i + 2] = t4[i].z;
+  }
+  return 0;
+  ABI_GUARD_END(return -1)
+}
+
+double renderPoseBatch(const float* poses, size_t poseCount, float* rgbOut) {
+  ABI_GUARD_BEGIN
+  State& s = st();
+  EyeState* e = current_eye();
+  if (!s.loaded || !e) throw std::runtime_error("renderPoseBatch: no compound eye selected");
+  if (poseCount == 0) return 0.0;
+  if (!poses) throw std::runtime_error("renderPoseBatch: null poses");
+  upload_eye(*e);
+  std::vector<Pose> ps(poseCount);
+  for (size_t k = 0; k < poseCount; ++k) {
+    Pose& p = ps[k];
+    const float* q = poses + 6 * k;
+    pose_reset(&p);
+    pose_rotate_around(&p, q[3], 1.0f, 0.0f, 0.0f);
+    pose_rotate_around(&p, q[4], 0.0f, 1.0f, 0.0f);
+    pose_rotate_around(&p, q[5], 0.0f, 0.0f, 1.0f);
+    pose_move(&p, q[0], q[1], q[2]);
+  }
+  EyeLaunch L = launch_of(*e, nullptr, (int)poseCount);
+  std::string err;
+  double ms = -1.0;
+  if (!s.batch.render(s.scene, L, ps, rgbOut, s.stream, &ms, &err)) throw std::runtime_error(err);
+  return ms;
+  ABI_GUARD_END(return -1.0)
+}
+
+int getSceneBVHInfo(int64_t* info) {
+  State& s = st();
+  if (!info || !s.loaded) return -1;
+  info[0] = s.scene.tri_count;
+  info[1] = s.scene.node_count;
+  info[2] = s.scene.stack_bound;
+  info[3] = s.scene.levels;
+  info[4] = s.scene.leaf_max;
+  info[5] = (int64_t)(s.scene.build_ms * 1000.0);
+  info[6] = 0;
+  info[7] = 0;
+  return 0;
+}
+
+size_t copySceneBVHNodes(void* dst, size_t bytes) {
+  ABI_GUARD_BEGIN
+  State& s = st();
+  if (!s.loaded || !dst) return 0;
+  const size_t n = std::min(bytes / sizeof(WideNode), (size_t)s.scene.node_count);
+  if (n) hip_check(hipMemcpy(dst, s.scene.nodes, n * sizeof(WideNode), hipMemcpyDeviceToHost), "readback");
+  return n * sizeof(WideNode);
+  ABI_GUARD_END(return 0)
+}
+
+size_t copySceneSortOrder(uint32_t* ids, uint32_t* codes, size_t count) {
+  ABI_GUARD_BEGIN
+  State& s = st();
+  if (!s.loaded) return 0;
+  const size_t n = std::min(count, (size_t)s.scene.tri_count);
+  if (n && ids) hip_check(hipMemcpy(ids, s.scene.sorted_ids, n * 4, hipMemcpyDeviceToHost), "readback");
+  if (n && codes) hip_check(hipMemcpy(codes, s.scene.codes, n * 4, hipMemcpyDeviceToHost), "readback");
+  return n;
+  ABI_GUARD_END(return 0)
+}
+
+int setSceneLeafSize(int leafSize) {
+  ABI_GUARD_BEGIN
+  State& s = st();
+  if (leafSize < 1 || leafSize > kLeafMaxLimit) throw std::runtime_error("setSceneLeafSize: leaf size must be 1..16");
+  s.leaf_max = leafSize;
+  if (s.loaded) build_scene_bvh();
+  return 0;
+  ABI_GUARD_END(return -1)
+}
+
+int64_t parseGlTFTriangles(const char* path, float* pos, float* col, uint32_t* mode, size_t maxTris) {
+  ABI_GUARD_BEGIN
+  if (!path) throw std::runtime_error("parseGlTFTriangles: null path");
+  HostScene hs;
+  std::string err;
+  if (!load_gltf(path, &hs, &err)) throw std::runtime_error(err);
+  const size_t n = hs.tri_count();
+  const size_t m = std::min(n, maxTris);
+  if (m) {
+    if (pos) memcpy(pos, hs.pos.data(), sizeof(float) * 9 * m);
+    if (col) memcpy(col, hs.col.data(), sizeof(float) * 9 * m);
+    if (mode) memcpy(mode, hs.mode.data(), sizeof(uint32_t) * m);
+  }
+  return (int64_t)n;
+  ABI_GUARD_END(return -1)
+}
+
+}  // extern "C"

@@ -1,0 +1,542 @@
+// On-GPU LBVH build for gfx950: Morton codes, LDS-staged stable radix sort,
+// Karras hierarchy, bottom-up refit, breadth-first collapse to a 4-wide BVH.
+// The rules are the ones documented in bvh_format.h; this file is the
+// parallel implementation.  Replaces the OptiX GAS/IAS build of
+// libEyeRenderer3/MulticamScene.cpp:1052-1428.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <string>
+#include <vector>
+
+#include "device_scene.h"
+#include "scan.h"
+
+namespace eye {
+namespace {
+
+#define LB_TRY(x)                                                                  \
+  do {                                                                             \
+    hipError_t e_ = (x);                                                           \
+    if (e_ != hipSuccess) {                                                        \
+      if (err) *err = std::string("lbvh: ") + #x + ": " + hipGetErrorString(e_);   \
+      return false;                                                                \
+    }                                                                              \
+  } while (0)
+
+constexpr int kSortThreads = 256;
+constexpr int kSortItems = 8;
+constexpr int kSortTile = kSortThreads * kSortItems;
+constexpr int kRadix = 256;
+constexpr int kWaves = kSortThreads / 64;
+
+// Deterministic min/max (ties return the second operand), identical on the CPU.
+__device__ __forceinline__ float dmin(float a, float b) { return a < b ? a : b; }
+__device__ __forceinline__ float dmax(float a, float b) { return a > b ? a : b; }
+
+// Order-preserving map of floats to unsigned keys (-0 sorts below +0).
+__device__ __forceinline__ uint32_t fkey(float f) {
+  const uint32_t b = __float_as_uint(f);
+  return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+__host__ __device__ inline float fkey_inv(uint32_t k) {
+  const uint32_t b = (k & 0x80000000u) ? (k & 0x7fffffffu) : ~k;
+  float f;
+  memcpy(&f, &b, 4);
+  return f;
+}
+
+__global__ void k_tri_bounds(const float* __restrict__ pos, int n, float* __restrict__ tb,
+                             float* __restrict__ cen, uint32_t* __restrict__ cbounds) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t klo[3] = {0xffffffffu, 0xffffffffu, 0xffffffffu};
+  uint32_t khi[3] = {0u, 0u, 0u};
+  if (i < n) {
+    const float* p = pos + 9 * (size_t)i;
+    for (int a = 0; a < 3; ++a) {
+      const float lo = dmin(dmin(p[a], p[3 + a]), p[6 + a]);
+      const float hi = dmax(dmax(p[a], p[3 + a]), p[6 + a]);
+      tb[6 * (size_t)i + a] = lo;
+      tb[6 * (size_t)i + 3 + a] = hi;
+      const float c = (lo + hi) * 0.5f;
+      cen[3 * (size_t)i + a] = c;
+      klo[a] = fkey(c);
+      khi[a] = fkey(c);
+    }
+  }
+  // wave reduction, then one atomic per wave and axis
+  for (int a = 0; a < 3; ++a) {
+    uint32_t lo = klo[a], hi = khi[a];
+    for (int m = 32; m >= 1; m >>= 1) {
+      const uint32_t olo = __shfl_xor(lo, m);
+      const uint32_t ohi = __shfl_xor(hi, m);
+      lo = olo < lo ? olo : lo;
+      hi = ohi > hi ? ohi : hi;
+    }
+    if ((threadIdx.x & 63) == 0) {
+      atomicMin(&cbounds[a], lo);
+      atomicMax(&cbounds[3 + a], hi);
+    }
+  }
+}
+
+__global__ void k_morton(const float* __restrict__ cen, int n, const uint32_t* __restrict__ cbounds,
+                         uint32_t* __restrict__ codes, uint32_t* __restrict__ ids) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float lo[3], hi[3];
+  for (int a = 0; a < 3; ++a) {
+    lo[a] = fkey_inv(cbounds[a]);
+    hi[a] = fkey_inv(cbounds[3 + a]);
+  }
+  codes[i] = morton30(cen[3 * (size_t)i], cen[3 * (size_t)i + 1], cen[3 * (size_t)i + 2], lo, hi);
+  ids[i] = (uint32_t)i;
+}
+
+// Per-tile digit histogram, written digit-major: hist[digit * tiles + tile].
+__global__ __launch_bounds__(kSortThreads) void k_sort_hist(const uint32_t* __restrict__ keys, int n,
+                                                            int shift, uint32_t* __restrict__ hist,
+                                                            int tiles) {
+  __shared__ uint32_t h[kRadix];
+  h[threadIdx.x] = 0;
+  __syncthreads();
+  const size_t base = (size_t)blockIdx.x * kSortTile;
+  for (int j = 0; j < kSortItems; ++j) {
+    const size_t idx = base + (size_t)j * kSortThreads + threadIdx.x;
+    if (idx < (size_t)n) atomicAdd(&h[(keys[idx] >> shift) & (kRadix - 1)], 1u);
+  }
+  __syncthreads();
+  hist[(size_t)threadIdx.x * tiles + blockIdx.x] = h[threadIdx.x];
+}
+
+// Stable scatter.  Items are taken in the order (item j, wave, lane); ranks
+// among equal digits inside a wave come from ballot matching, across waves
+// and items from LDS running counters, across tiles from the scanned
+// histogram.
+__global__ __launch_bounds__(kSortThreads) void k_sort_scatter(
+    const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin, uint32_t* __restrict__ kout,
+    uint32_t* __restrict__ vout, int n, int shift, const uint32_t* __restrict__ offs, int tiles) {
+  __shared__ uint32_t wave_cnt[kWaves][kRadix];
+  __shared__ uint32_t wave_off[kWaves][kRadix];
+  __shared__ uint32_t run_base[kRadix];
+  __shared__ uint32_t g_off[kRadix];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int w = tid >> 6;
+  run_base[tid] = 0;
+  g_off[tid] = offs[(size_t)tid * tiles + blockIdx.x];
+  const uint64_t lt_mask = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+  const size_t base = (size_t)blockIdx.x * kSortTile;
+  for (int j = 0; j < kSortItems; ++j) {
+    const size_t idx = base + (size_t)j * kSortThreads + tid;
+    const bool valid = idx < (size_t)n;
+    const uint32_t key = valid ? kin[idx] : 0u;
+    const uint32_t val = valid ? vin[idx] : 0u;
+    const uint32_t d = (key >> shift) & (kRadix - 1);
+    for (int k = 0; k < kWaves; ++k) wave_cnt[k][tid] = 0;
+    __syncthreads();
+    uint64_t peers = __ballot(valid);
+    for (int b = 0; b < 8; ++b) {
+      const bool bit = (d >> b) & 1u;
+      const uint64_t m = __ballot(bit);
+      peers &= bit ? m : ~m;
+    }
+    uint32_t rank = 0;
+    if (valid) {
+      rank = (uint32_t)__popcll(peers & lt_mask);
+      const int leader = 63 - __clzll((long long)peers);
+      if (leader == lane) wave_cnt[w][d] = (uint32_t)__popcll(peers);
+    }
+    __syncthreads();
+    {
+      uint32_t run = run_base[tid];
+      for (int k = 0; k < kWaves; ++k) {
+        const uint32_t c = wave_cnt[k][tid];
+        wave_off[k][tid] = run;
+        run += c;
+      }
+      run_base[tid] = run;
+    }
+    __syncthreads();
+    if (valid) {
+      const uint32_t pos = g_off[d] + wave_off[w][d] + rank;
+      if (pos < (uint32_t)n) {  // always true for a consistent histogram; never write out of bounds
+        kout[pos] = key;
+        vout[pos] = val;
+      }
+    }
+  }
+}
+
+__global__ void k_karras(const uint32_t* __restrict__ codes, int n, int32_t* __restrict__ left,
+                         int32_t* __restrict__ right, int32_t* __restrict__ parent,
+                         int32_t* __restrict__ range) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {  // leaf ranges
+    range[2 * (size_t)(n - 1 + i)] = i;
+    range[2 * (size_t)(n - 1 + i) + 1] = i;
+  }
+  if (i >= n - 1) return;
+  int l, r, first, last;
+  bool ll, rl;
+  karras_node(codes, n, i, &l, &ll, &r, &rl, &first, &last);
+  const int lc = ll ? (n - 1 + l) : l;
+  const int rc = rl ? (n - 1 + r) : r;
+  left[i] = lc;
+  right[i] = rc;
+  parent[lc] = i;
+  parent[rc] = i;
+  range[2 * (size_t)i] = first;
+  range[2 * (size_t)i + 1] = last;
+  if (i == 0) parent[0] = -1;
+}
+
+__global__ void k_refit(const uint32_t* __restrict__ ids, const float* __restrict__ tb, int n,
+                        const int32_t* __restrict__ left, const int32_t* __restrict__ right,
+                        const int32_t* __restrict__ parent, float* bounds, uint32_t* flags) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const int leaf = n - 1 + k;
+  const uint32_t id = ids[k];
+  for (int a = 0; a < 6; ++a) bounds[6 * (size_t)leaf + a] = tb[6 * (size_t)id + a];
+  if (n == 1) return;
+  __threadfence();
+  int p = parent[leaf];
+  // a radix tree over 30-bit codes tie-broken by 27-bit indices is at most 58
+  // levels deep; the cap only guards against a corrupted parent array
+  for (int guard = 0; p >= 0 && p < n - 1 && guard < 128; ++guard) {
+    const uint32_t old = atomicAdd(&flags[p], 1u);
+    if (old == 0) return;
+    __threadfence();
+    const volatile float* vb = bounds;
+    const int l = left[p], r = right[p];
+    float b[6];
+    for (int a = 0; a < 3; ++a) {
+      b[a] = dmin(vb[6 * (size_t)l + a], vb[6 * (size_t)r + a]);
+      b[3 + a] = dmax(vb[6 * (size_t)l + 3 + a], vb[6 * (size_t)r + 3 + a]);
+    }
+    for (int a = 0; a < 6; ++a) bounds[6 * (size_t)p + a] = b[a];
+    __threadfence();
+    p = parent[p];
+  }
+}
+
+__global__ void k_collapse_count(const int32_t* __restrict__ srcs, int count, BinaryTreeView t,
+                                 int leaf_max, uint32_t* __restrict__ internal_count) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= count) return;
+  int ch[4];
+  const int c = collapse_children(t, srcs[j], leaf_max, ch);
+  uint32_t k = 0;
+  for (int s = 0; s < c; ++s) k += wide_child_is_internal(t, ch[s], leaf_max) ? 1u : 0u;
+  internal_count[j] = k;
+}
+
+__global__ void k_collapse_emit(const int32_t* __restrict__ srcs, int count, BinaryTreeView t,
+                                int leaf_max, const uint32_t* __restrict__ offs, int base,
+                                int next_base, int32_t* __restrict__ next_srcs,
+                                WideNode* __restrict__ nodes, int32_t* __restrict__ depth,
+                                int32_t* __restrict__ stack_bound) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= count) return;
+  int ch[4];
+  const int c = collapse_children(t, srcs[j], leaf_max, ch);
+  const int w = base + j;
+  const int dw = depth[w];
+  int k_internal = 0;
+  for (int s = 0; s < c; ++s) k_internal += wide_child_is_internal(t, ch[s], leaf_max) ? 1 : 0;
+  WideNode node;
+  int next = 0;
+  for (int s = 0; s < 4; ++s) {
+    if (s < c) {
+      const int b = ch[s];
+      const float* bb = t.bounds + 6 * (size_t)b;
+      node.lo_x[s] = bb[0]; node.lo_y[s] = bb[1]; node.lo_z[s] = bb[2];
+      node.hi_x[s] = bb[3]; node.hi_y[s] = bb[4]; node.hi_z[s] = bb[5];
+      if (wide_child_is_internal(t, b, leaf_max)) {
+        const int slot = (int)offs[j] + next;
+        ++next;
+        node.child[s] = next_base + slot;
+        next_srcs[slot] = b;
+        depth[next_base + slot] = dw + (k_internal - 1);
+      } else {
+        node.child[s] = encode_leaf((uint32_t)t.range[2 * (size_t)b], (uint32_t)bin_size(t, b));
+      }
+    } else {
+      node.lo_x[s] = node.hi_x[s] = kEmptyBox;
+      node.lo_y[s] = node.hi_y[s] = kEmptyBox;
+      node.lo_z[s] = node.hi_z[s] = kEmptyBox;
+      node.child[s] = kChildEmpty;
+    }
+    node.pad[s] = 0;
+  }
+  nodes[w] = node;
+  atomicMax(stack_bound, dw + (k_internal > 1 ? k_internal - 1 : 0));
+}
+
+__global__ void k_gather(const float* __restrict__ pos, const float* __restrict__ col,
+                         const uint32_t* __restrict__ mode, const uint32_t* __restrict__ ids, int n,
+                         float4* __restrict__ tris, float4* __restrict__ cols) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const uint32_t id = ids[k];
+  const float* p = pos + 9 * (size_t)id;
+  const float* c = col + 9 * (size_t)id;
+  tris[3 * (size_t)k + 0] = make_float4(p[0], p[1], p[2], __uint_as_float(id));
+  tris[3 * (size_t)k + 1] = make_float4(p[3], p[4], p[5], 0.0f);
+  tris[3 * (size_t)k + 2] = make_float4(p[6], p[7], p[8], 0.0f);
+  cols[3 * (size_t)k + 0] = make_float4(c[0], c[1], c[2], __uint_as_float(mode[id]));
+  cols[3 * (size_t)k + 1] = make_float4(c[3], c[4], c[5], 0.0f);
+  cols[3 * (size_t)k + 2] = make_float4(c[6], c[7], c[8], 0.0f);
+}
+
+inline int blocks_for(size_t n, int threads) { return (int)((n + threads - 1) / threads); }
+
+// Host check of the finished structure before anything indexes through it:
+// the sort order is a permutation, every internal child index points forward
+// to a node referenced exactly once, and the leaves cover every triangle
+// exactly once.  A violation is reported as an error instead of letting a
+// kernel read out of bounds.
+bool validate_structure(const uint32_t* d_ids, int n, const WideNode* d_nodes, int node_count, hipStream_t stream,
+                        std::string* err) {
+  std::vector<uint32_t> ids((size_t)n);
+  std::vector<WideNode> nodes((size_t)node_count);
+  hipError_t e = hipMemcpyAsync(ids.data(), d_ids, sizeof(uint32_t) * (size_t)n, hipMemcpyDeviceToHost, stream);
+  if (e == hipSuccess && node_count > 0)
+    e = hipMemcpyAsync(nodes.data(), d_nodes, sizeof(WideNode) * (size_t)node_count, hipMemcpyDeviceToHost, stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(stream);
+  if (e != hipSuccess) {
+    if (err) *err = std::string("lbvh: validation readback failed: ") + hipGetErrorString(e);
+    return false;
+  }
+  std::vector<uint8_t> seen((size_t)n, 0);
+  for (uint32_t v : ids) {
+    if (v >= (uint32_t)n || seen[v]) {
+      if (err) *err = "lbvh: sorted order is not a permutation";
+      return false;
+    }
+    seen[v] = 1;
+  }
+  std::vector<uint32_t> refs((size_t)node_count, 0);
+  std::vector<uint8_t> covered((size_t)n, 0);
+  for (int w = 0; w < node_count; ++w)
+    for (int s = 0; s < 4; ++s) {
+      const int32_t c = nodes[(size_t)w].child[s];
+      if (c == kChildEmpty) continue;
+      if (child_is_leaf(c)) {
+        const uint32_t f = leaf_first(c), k = leaf_count(c);
+        if ((uint64_t)f + k > (uint64_t)n) {
+          if (err) *err = "lbvh: leaf range out of bounds";
+          return false;
+        }
+        for (uint32_t q = f; q < f + k; ++q) {
+          if (covered[q]) {
+            if (err) *err = "lbvh: triangle referenced by two leaves";
+            return false;
+          }
+          covered[q] = 1;
+        }
+      } else {
+        if (c <= w || c >= node_count || refs[(size_t)c]++) {
+          if (err) *err = "lbvh: bad internal child index";
+          return false;
+        }
+      }
+    }
+  for (int w = 1; w < node_count; ++w)
+    if (refs[(size_t)w] != 1) {
+      if (err) *err = "lbvh: unreachable wide node";
+      return false;
+    }
+  for (int q = 0; q < n; ++q)
+    if (!covered[(size_t)q]) {
+      if (err) *err = "lbvh: triangle not covered by any leaf";
+      return false;
+    }
+  return true;
+}
+
+}  // namespace
+
+void free_device_scene(DeviceScene* s) {
+  if (s->nodes) (void)hipFree(s->nodes);
+  if (s->tris) (void)hipFree(s->tris);
+  if (s->cols) (void)hipFree(s->cols);
+  if (s->sorted_ids) (void)hipFree(s->sorted_ids);
+  if (s->codes) (void)hipFree(s->codes);
+  *s = DeviceScene();
+}
+
+bool build_device_scene(const float* d_pos, const float* d_col, const uint32_t* d_mode, int n,
+                        int leaf_max, hipStream_t stream, DeviceScene* out, std::string* err) {
+  const auto t0 = std::chrono::steady_clock::now();
+  free_device_scene(out);
+  out->leaf_max = leaf_max;
+  out->tri_count = n;
+  if (leaf_max < 1 || leaf_max > kLeafMaxLimit) {
+    if (err) *err = "lbvh: leaf_max must be in [1, 16]";
+    return false;
+  }
+  if (n < 0 || n >= (1 << 27)) {
+    if (err) *err = "lbvh: triangle count must be below 2^27";
+    return false;
+  }
+  if (n == 0) {
+    out->node_count = 0;
+    return true;
+  }
+  const int nb = 2 * n - 1;  // binary nodes, combined indexing
+  const int tiles = blocks_for(n, kSortTile);
+  float* tb = nullptr;
+  float* cen = nullptr;
+  uint32_t* cbounds = nullptr;
+  uint32_t *codes_a = nullptr, *codes_b = nullptr, *ids_a = nullptr, *ids_b = nullptr;
+  uint32_t* hist = nullptr;
+  int32_t *left = nullptr, *right = nullptr, *parent = nullptr, *range = nullptr;
+  float* bounds = nullptr;
+  uint32_t* flags = nullptr;
+  int32_t *srcs_a = nullptr, *srcs_b = nullptr, *depth = nullptr, *stack_bound = nullptr;
+  uint32_t *icount = nullptr, *ioffs = nullptr;
+  ScanScratch scan;
+  bool ok = false;
+  const int max_nodes = n > 1 ? n - 1 : 1;
+  std::vector<uint32_t> init_bounds = {0xffffffffu, 0xffffffffu, 0xffffffffu, 0u, 0u, 0u};
+  int node_total = 0, level_count = 0, host_sb = 0;
+  do {
+#define LB_STEP(x)                                                                         \
+  {                                                                                        \
+    hipError_t e_ = (x);                                                                   \
+    if (e_ != hipSuccess) {                                                                \
+      if (err) *err = std::string("lbvh: ") + #x + ": " + hipGetErrorString(e_);           \
+      break;                                                                               \
+    }                                                                                      \
+  }
+    bool fail = true;
+    do {
+      LB_STEP(hipMalloc(&tb, sizeof(float) * 6 * (size_t)n));
+      LB_STEP(hipMalloc(&cen, sizeof(float) * 3 * (size_t)n));
+      LB_STEP(hipMalloc(&cbounds, sizeof(uint32_t) * 6));
+      LB_STEP(hipMalloc(&codes_a, sizeof(uint32_t) * (size_t)n));
+      LB_STEP(hipMalloc(&codes_b, sizeof(uint32_t) * (size_t)n));
+      LB_STEP(hipMalloc(&ids_a, sizeof(uint32_t) * (size_t)n));
+      LB_STEP(hipMalloc(&ids_b, sizeof(uint32_t) * (size_t)n));
+      LB_STEP(hipMalloc(&hist, sizeof(uint32_t) * (size_t)kRadix * tiles));
+      LB_STEP(hipMalloc(&left, sizeof(int32_t) * (size_t)max_nodes));
+      LB_STEP(hipMalloc(&right, sizeof(int32_t) * (size_t)max_nodes));
+      LB_STEP(hipMalloc(&parent, sizeof(int32_t) * (size_t)nb));
+      LB_STEP(hipMalloc(&range, sizeof(int32_t) * 2 * (size_t)nb));
+      LB_STEP(hipMalloc(&bounds, sizeof(float) * 6 * (size_t)nb));
+      LB_STEP(hipMalloc(&flags, sizeof(uint32_t) * (size_t)max_nodes));
+      LB_STEP(hipMalloc(&srcs_a, sizeof(int32_t) * (size_t)max_nodes));
+      LB_STEP(hipMalloc(&srcs_b, sizeof(int32_t) * (size_t)max_nodes));
+      LB_STEP(hipMalloc(&depth, sizeof(int32_t) * (size_t)max_nodes));
+      LB_STEP(hipMalloc(&stack_bound, sizeof(int32_t)));
+      LB_STEP(hipMalloc(&icount, sizeof(uint32_t) * (size_t)max_nodes));
+      LB_STEP(hipMalloc(&ioffs, sizeof(uint32_t) * ((size_t)max_nodes + 1)));
+      LB_STEP(hipMalloc(&out->nodes, sizeof(WideNode) * (size_t)max_nodes));
+      LB_STEP(hipMalloc(&out->tris, sizeof(float4) * 3 * (size_t)n));
+      LB_STEP(hipMalloc(&out->cols, sizeof(float4) * 3 * (size_t)n));
+      LB_STEP(hipMalloc(&out->sorted_ids, sizeof(uint32_t) * (size_t)n));
+      LB_STEP(hipMalloc(&out->codes, sizeof(uint32_t) * (size_t)n));
+      if (!scan_reserve(&scan, (size_t)kRadix * tiles > (size_t)max_nodes + 1 ? (size_t)kRadix * tiles
+                                                                              : (size_t)max_nodes + 1,
+                        err))
+        break;
+      fail = false;
+    } while (0);
+    if (fail) break;
+    fail = true;
+    do {
+      LB_STEP(hipMemcpyAsync(cbounds, init_bounds.data(), sizeof(uint32_t) * 6, hipMemcpyHostToDevice, stream));
+      LB_STEP(hipMemsetAsync(flags, 0, sizeof(uint32_t) * (size_t)max_nodes, stream));
+      LB_STEP(hipMemsetAsync(stack_bound, 0, sizeof(int32_t), stream));
+      LB_STEP(hipMemsetAsync(depth, 0, sizeof(int32_t), stream));
+      hipLaunchKernelGGL(k_tri_bounds, dim3(blocks_for(n, 256)), dim3(256), 0, stream, d_pos, n, tb, cen, cbounds);
+      hipLaunchKernelGGL(k_morton, dim3(blocks_for(n, 256)), dim3(256), 0, stream, cen, n, cbounds, codes_a, ids_a);
+      // 30-bit keys: passes at bits 0, 8, 16, 24
+      uint32_t *kin = codes_a, *kout = codes_b, *vin = ids_a, *vout = ids_b;
+      bool sort_ok = true;
+      for (int shift = 0; shift < 30; shift += 8) {
+        hipLaunchKernelGGL(k_sort_hist, dim3(tiles), dim3(kSortThreads), 0, stream, kin, n, shift, hist, tiles);
+        if (!scan_exclusive_u32(&scan, hist, (size_t)kRadix * tiles, stream, err)) { sort_ok = false; break; }
+        hipLaunchKernelGGL(k_sort_scatter, dim3(tiles), dim3(kSortThreads), 0, stream, kin, vin, kout, vout, n,
+                           shift, hist, tiles);
+        uint32_t* t = kin; kin = kout; kout = t;
+        t = vin; vin = vout; vout = t;
+      }
+      if (!sort_ok) break;
+      // 4 passes: result back in the *_a buffers
+      LB_STEP(hipMemcpyAsync(out->codes, kin, sizeof(uint32_t) * (size_t)n, hipMemcpyDeviceToDevice, stream));
+      LB_STEP(hipMemcpyAsync(out->sorted_ids, vin, sizeof(uint32_t) * (size_t)n, hipMemcpyDeviceToDevice, stream));
+      hipLaunchKernelGGL(k_karras, dim3(blocks_for(n, 256)), dim3(256), 0, stream, kin, n, left, right, parent, range);
+      if (n == 1) LB_STEP(hipMemsetAsync(parent, 0xff, sizeof(int32_t), stream));
+      hipLaunchKernelGGL(k_refit, dim3(blocks_for(n, 256)), dim3(256), 0, stream, vin, tb, n, left, right, parent,
+                         bounds, flags);
+      LB_STEP(hipGetLastError());
+      BinaryTreeView tv;
+      tv.left = left; tv.right = right; tv.bounds = bounds; tv.range = range; tv.n = n;
+      // breadth-first collapse
+      int32_t root = 0;
+      LB_STEP(hipMemcpyAsync(srcs_a, &root, sizeof(int32_t), hipMemcpyHostToDevice, stream));
+      int level_n = 1, base = 0;
+      int32_t *cur = srcs_a, *nxt = srcs_b;
+      bool lvl_ok = true;
+      while (level_n > 0) {
+        hipLaunchKernelGGL(k_collapse_count, dim3(blocks_for(level_n, 256)), dim3(256), 0, stream, cur, level_n, tv,
+                           leaf_max, icount);
+        uint32_t total = 0;
+        if (!scan_exclusive_u32_total(&scan, icount, ioffs, (size_t)level_n, stream, &total, err)) {
+          lvl_ok = false;
+          break;
+        }
+        const int next_base = base + level_n;
+        if ((size_t)next_base + total > (size_t)max_nodes) {
+          if (err) *err = "lbvh: wide node count exceeds the binary bound";
+          lvl_ok = false;
+          break;
+        }
+        hipLaunchKernelGGL(k_collapse_emit, dim3(blocks_for(level_n, 256)), dim3(256), 0, stream, cur, level_n, tv,
+                           leaf_max, ioffs, base, next_base, nxt, out->nodes, depth, stack_bound);
+        base = next_base;
+        level_n = (int)total;
+        int32_t* t = cur; cur = nxt; nxt = t;
+        ++level_count;
+      }
+      if (!lvl_ok) break;
+      node_total = base;
+      if (!validate_structure(vin, n, out->nodes, node_total, stream, err)) break;
+      hipLaunchKernelGGL(k_gather, dim3(blocks_for(n, 256)), dim3(256), 0, stream, d_pos, d_col, d_mode, vin, n,
+                         out->tris, out->cols);
+      LB_STEP(hipGetLastError());
+      uint32_t cb[6];
+      LB_STEP(hipMemcpyAsync(&host_sb, stack_bound, sizeof(int32_t), hipMemcpyDeviceToHost, stream));
+      LB_STEP(hipMemcpyAsync(cb, cbounds, sizeof(cb), hipMemcpyDeviceToHost, stream));
+      LB_STEP(hipStreamSynchronize(stream));
+      for (int a = 0; a < 3; ++a) {
+        out->centroid_lo[a] = fkey_inv(cb[a]);
+        out->centroid_hi[a] = fkey_inv(cb[3 + a]);
+      }
+      fail = false;
+    } while (0);
+    if (fail) break;
+    ok = true;
+#undef LB_STEP
+  } while (0);
+  for (void* p : {(void*)tb, (void*)cen, (void*)cbounds, (void*)codes_a, (void*)codes_b, (void*)ids_a,
+                  (void*)ids_b, (void*)hist, (void*)left, (void*)right, (void*)parent, (void*)range,
+                  (void*)bounds, (void*)flags, (void*)srcs_a, (void*)srcs_b, (void*)depth, (void*)stack_bound,
+                  (void*)icount, (void*)ioffs})
+    if (p) (void)hipFree(p);
+  scan_release(&scan);
+  if (!ok) {
+    free_device_scene(out);
+    return false;
+  }
+  out->node_count = node_total;
+  out->stack_bound = host_sb;
+  out->levels = level_count;
+  out->build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return true;
+}
+
+}  // namespace eye

@@ -1,0 +1,185 @@
+/*
+ * libEyeRenderer3 C ABI of the MI355X compound-eye renderer.
+ *
+ * The first block keeps the reference's exported symbols with identical names,
+ * argument meaning and C signatures (libEyeRenderer3/libEyeRenderer.h:16-67),
+ * so the reference's ctypes helpers
+ * (python-examples/eyeRendererHelperFunctions.py:40-71) drive this library
+ * unchanged.  Differences in behaviour, all additive:
+ *   - headless: no window is created at load time, displayFrame is a no-op;
+ *   - nothing throws across the ABI: a failing call records a message that
+ *     eyeRendererGetLastError() returns and leaves the renderer usable;
+ *   - errors the reference reports only on stderr keep their silent / no-op
+ *     semantics (missing .eye file, compound setters on a regular camera).
+ * The second block holds the additions: float readback, per-ommatidium sample
+ * counts, random-stream control, batched rendering of many poses, ray-level
+ * access, acceleration-structure inspection and the learned sampler.
+ *
+ * Plain C types only: pointers, sizes, floats.  The library owns every
+ * returned pointer (valid until the next call that changes the same state).
+ */
+#ifndef EYE_RENDERER_MI355X_H
+#define EYE_RENDERER_MI355X_H
+
+#include <stddef.h>
+#include <stdint.h>
+#ifndef __cplusplus
+#include <stdbool.h>
+#endif
+
+/* libEyeRenderer3/libEyeRenderer.h:8-14 (32 bytes). */
+typedef struct OmmatidiumPacket {
+  float posX, posY, posZ;
+  float dirX, dirY, dirZ;
+  float acceptanceAngle;
+  float focalpointOffset;
+} OmmatidiumPacket;
+
+/* Stand-in for CUDA's float3 in the two struct-returning calls: 12 bytes,
+ * returned in registers exactly like the reference's float3
+ * (eyeRendererHelperFunctions.py:6-9 declares it as three c_float). */
+typedef struct EyeFloat3 {
+  float x, y, z;
+} EyeFloat3;
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+#if defined(__GNUC__)
+#pragma GCC visibility push(default)
+#endif
+
+/* ---- Configuration (libEyeRenderer.h:18-25, libEyeRenderer.cpp:211-295) ---- */
+void setVerbosity(bool v);                  /* libEyeRenderer.h:19 */
+void loadGlTFscene(const char* filepath);   /* libEyeRenderer.h:20, libEyeRenderer.cpp:215-220 */
+void stop(void);                            /* libEyeRenderer.h:21, libEyeRenderer.cpp:289-295 */
+void setRenderSize(int w, int h);           /* libEyeRenderer.h:22, libEyeRenderer.cpp:221-228 */
+double renderFrame(void);                   /* libEyeRenderer.h:23, libEyeRenderer.cpp:229-242: host wall ms */
+void displayFrame(void);                    /* libEyeRenderer.h:24: headless no-op */
+void saveFrameAs(char* ppmFilename);        /* libEyeRenderer.h:25, libEyeRenderer.cpp:259-269: P6, flipped */
+unsigned char* getFramePointer(void);       /* libEyeRenderer.h:26, libEyeRenderer.cpp:270-275: h*w*4 bytes */
+
+/* ---- Camera control (libEyeRenderer.h:29-50, libEyeRenderer.cpp:306-388) ---- */
+size_t getCameraCount(void);
+void nextCamera(void);
+void previousCamera(void);
+size_t getCurrentCameraIndex(void);
+const char* getCurrentCameraName(void);
+void gotoCamera(int index);                 /* index wraps modulo the camera count (MulticamScene.cpp:928-933) */
+bool gotoCameraByName(char* name);
+void setCameraPosition(float x, float y, float z);
+/* The reference takes float& x, y, z; C++ references are pointers in the ABI. */
+void getCameraPosition(float* x, float* y, float* z);
+void setCameraLocalSpace(float lxx, float lxy, float lxz, float lyx, float lyy, float lyz, float lzx,
+                         float lzy, float lzz);
+void rotateCameraAround(float angle, float axisX, float axisY, float axisZ);
+void rotateCameraLocallyAround(float angle, float axisX, float axisY, float axisZ);
+void translateCamera(float x, float y, float z);
+void translateCameraLocally(float x, float y, float z);
+void resetCameraPose(void);
+/* Rotations about world X, then Y, then Z (radians), then translation
+ * (libEyeRenderer.cpp:380-388). */
+void setCameraPose(float posX, float posY, float posZ, float rotX, float rotY, float rotZ);
+
+/* ---- Compound eye (libEyeRenderer.h:52-62, libEyeRenderer.cpp:393-468) ---- */
+bool isCompoundEyeActive(void);
+void setCurrentEyeSamplesPerOmmatidium(int s);        /* clamps to >= 1, restarts the random streams */
+int getCurrentEyeSamplesPerOmmatidium(void);          /* -1 when the camera is not a compound eye */
+void changeCurrentEyeSamplesPerOmmatidiumBy(int s);
+size_t getCurrentEyeOmmatidialCount(void);            /* 0 when the camera is not a compound eye */
+void setOmmatidia(OmmatidiumPacket* omms, size_t count);
+const char* getCurrentEyeDataPath(void);
+void setCurrentEyeShaderName(char* name);             /* projection: single_dimension_fast, single_dimension,
+                                                         raw_ommatidial_samples */
+
+/* ---- Scene queries (libEyeRenderer.h:64-66, MulticamScene.cpp:1757-1818) ---- */
+bool isInsideHitGeometry(float x, float y, float z, char* name);
+EyeFloat3 getGeometryMaxBounds(char* name);
+EyeFloat3 getGeometryMinBounds(char* name);
+
+/* libEyeRenderer.h:69 declares this with C++ linkage and returns the GLFW
+ * window; headless here, so it is exported with C linkage and returns NULL. */
+void* getWindowPointer(void);
+
+/* ======================= Additions (no reference counterpart) =============== */
+
+/* Message of the last failed call ("" when none); cleared by the next successful
+ * loadGlTFscene or renderFrame. */
+const char* eyeRendererGetLastError(void);
+
+/* Linear (pre-gamma) RGB of every ommatidium of the last compound render,
+ * 3 floats each; copies min(count, ommatidia) entries and returns that number. */
+size_t getOmmatidialColours(float* rgb, size_t count);
+
+/* Per-ommatidium sample counts for the current eye (count must equal the
+ * ommatidial count; count == 0 returns to the uniform
+ * setCurrentEyeSamplesPerOmmatidium value).  Counts of 0 render black. */
+void setCurrentEyeSampleCounts(const uint32_t* counts, size_t count);
+/* Total samples per frame of the current eye. */
+uint64_t getCurrentEyeTotalSamples(void);
+
+/* Random streams: key (seed, eye index), counter (ommatidium, sample, frame,
+ * pose).  The frame index of the current eye advances after every render and
+ * returns to 0 where the reference re-seeds (sample count or ommatidial count
+ * changes). */
+void setRandomSeed(uint32_t seed);
+uint32_t getCurrentEyeFrameIndex(void);
+void setCurrentEyeFrameIndex(uint32_t frame);
+uint32_t getCurrentEyeIndex(void);   /* index among the scene's compound eyes, or 0xffffffff */
+
+/* Pose of the current camera: 12 floats (position, local X, Y, Z axes), the
+ * posed record the reference copies to the device (GenericCameraDataTypes.h:31-42).
+ * Returns 0, or -1 when there is no camera. */
+int getCameraPose(float* out12);
+
+/* GPU milliseconds of the last renderFrame, measured with HIP events on the
+ * render stream: out4 = (trace kernel, reduction kernel, projection kernel,
+ * sum).  Returns 0. */
+int getLastRenderTimings(double* out4);
+/* Traversal work of the rays the next renderFrame() of the current eye will
+ * trace: out3 = (BVH nodes visited, triangles tested, rays that hit).
+ * Returns 0, or -1 on error. */
+int getCurrentEyeTraversalStats(uint64_t* out3);
+
+/* Rays the next renderFrame() of the current eye will trace, 8 floats each
+ * (origin xyz, tmin, direction xyz, tmax), ray order (ommatidium, sample).
+ * Returns the number written (at most `count`). */
+size_t getCurrentEyeRays(float* rays, size_t count);
+/* Per-sample radiance of the last render of the current eye, 4 floats each
+ * (r, g, b, hit distance or -1 for a miss), same order. */
+size_t getCurrentEyeSamples(float* samples, size_t count);
+
+/* Closest hits of arbitrary rays (8 floats each as above) against the loaded
+ * scene: primIds[i] = scene triangle index or -1, tbb[i*3..] = (t, b1, b2).
+ * Returns 0 on success, -1 on error. */
+int traceRays(const float* rays, size_t count, int32_t* primIds, float* tbb);
+
+/* Batched rendering of the current eye at many poses in one launch sequence:
+ * poses[6*k..] = (posX, posY, posZ, rotX, rotY, rotZ) with setCameraPose
+ * semantics; rgbOut receives 3 floats per (pose, ommatidium), pose-major.
+ * Pose k uses counter word 3 = k.  Returns GPU milliseconds, or -1 on error. */
+double renderPoseBatch(const float* poses, size_t poseCount, float* rgbOut);
+
+/* Acceleration-structure inspection: info[0..7] = triangle count, node count,
+ * stack bound, levels, leaf size, build ms * 1000, 0, 0.  Returns 0 or -1. */
+int getSceneBVHInfo(int64_t* info);
+/* Copies the 4-wide node array (128 bytes per node); returns bytes copied. */
+size_t copySceneBVHNodes(void* dst, size_t bytes);
+/* Copies sorted triangle ids and Morton codes (n entries each). */
+size_t copySceneSortOrder(uint32_t* ids, uint32_t* codes, size_t count);
+/* Rebuilds the BVH of the loaded scene with another leaf size (1..16). */
+int setSceneLeafSize(int leafSize);
+
+/* Host-only glTF parse (no GPU): world-space triangles (9 floats), colours
+ * (9 floats) and colour modes of `path`.  Returns the triangle count (call
+ * with maxTris = 0 to query), or -1 on error. */
+int64_t parseGlTFTriangles(const char* path, float* pos, float* col, uint32_t* mode, size_t maxTris);
+
+#if defined(__GNUC__)
+#pragma GCC visibility pop
+#endif
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* EYE_RENDERER_MI355X_H */

@@ -1,0 +1,223 @@
+"""GPU parity of the HIP renderer against the CPU oracle, through the C ABI.
+
+Bar: rays, hit ids, hit distances, per-sample colours and per-ommatidium
+colours (canonical wave order) are bit-identical to the oracle; colours are
+within 1e-5 relative of the reference's sequential summation order; the BVH
+node array is byte-identical to the oracle's CPU build of the same codes.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from compound_ray_amd import eyerenderer as er
+from compound_ray_amd import scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def load(lib, path):
+    lib.loadGlTFscene(str(path).encode())
+    err = er.last_error(lib)
+    assert err == "", err
+    er.gotoFirstCompoundEye(lib)
+
+
+def setup_eye(lib, cfg, omm=None, spp=32, pose6=None):
+    load(lib, cfg["gltf"])
+    if omm is not None:
+        er.set_ommatidia_np(lib, omm)
+    lib.setCurrentEyeSamplesPerOmmatidium(spp)
+    p = cfg["pose"] if pose6 is None else pose6
+    lib.setCameraPose(*[float(v) for v in p])
+
+
+def rel_close(a, b, rtol):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return np.all(np.abs(a - b) <= rtol * np.maximum(np.abs(b), 1e-12))
+
+
+def test_pose_matches_oracle(lib, config1):
+    setup_eye(lib, config1)
+    got = er.pose_np(lib)
+    want = oracle.pose12(config1["pose"])
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+
+
+def test_bvh_bit_exact(lib, config1, oracle_c1):
+    load(lib, config1["gltf"])
+    info = er.bvh_info(lib)
+    oinfo = oracle_c1.info()
+    assert info["tris"] == oinfo["tris"] == config1["scene"].tri_count()
+    assert info["nodes"] == oinfo["nodes"]
+    assert info["stack_bound"] == oinfo["stack_bound"]
+    ids, codes = er.sort_order_np(lib)
+    oids, ocodes = oracle_c1.order()
+    assert np.array_equal(codes, ocodes)
+    assert np.array_equal(ids, oids)
+    assert np.array_equal(er.bvh_nodes_np(lib), oracle_c1.nodes())
+
+
+@pytest.mark.parametrize("leaf", [1, 2, 7, 16])
+def test_bvh_leaf_sizes(lib, config1, leaf):
+    load(lib, config1["gltf"])
+    assert lib.setSceneLeafSize(leaf) == 0
+    o = oracle.OracleScene.from_scene(config1["scene"], leaf_size=leaf)
+    assert np.array_equal(er.bvh_nodes_np(lib), o.nodes())
+    setup_eye(lib, config1, omm=config1["eyes"][0][:128], spp=8)
+    assert lib.setSceneLeafSize(leaf) == 0
+    assert lib.renderFrame() >= 0
+    want = o.render(config1["eyes"][0][:128], er.pose_np(lib), spp=8)["colour_gpu_order"][0]
+    assert np.array_equal(er.colours_np(lib), want)
+    assert lib.setSceneLeafSize(4) == 0
+
+
+def test_rays_bit_exact(lib, config1):
+    setup_eye(lib, config1, spp=32)
+    rays = er.rays_np(lib)
+    want = oracle.generate_rays(config1["eyes"][0], er.pose_np(lib), spp=32, seed=42,
+                                eye=lib.getCurrentEyeIndex(), frame=lib.getCurrentEyeFrameIndex())
+    assert rays.shape == (1000 * 32, 8)
+    assert np.array_equal(rays.view(np.uint32), want.view(np.uint32))
+
+
+def test_hits_match_oracle(lib, config1, oracle_c1):
+    setup_eye(lib, config1, spp=32)
+    rays = er.rays_np(lib)
+    ids, tbb = er.trace_np(lib, rays)
+    oids, otbb = oracle_c1.trace(rays, bvh=True)
+    agree = np.mean(ids == oids)
+    assert agree >= 0.9999, agree
+    assert np.array_equal(ids, oids)
+    hit = ids >= 0
+    assert np.array_equal(tbb[hit].view(np.uint32), otbb[hit].view(np.uint32))
+    # and against brute force over every triangle on a subset
+    bids, btbb = oracle_c1.trace(rays[:3000], bvh=False)
+    assert np.array_equal(ids[:3000], bids)
+
+
+def test_render_bit_exact_c1(lib, config1, oracle_c1):
+    setup_eye(lib, config1, spp=32)
+    ms = lib.renderFrame()
+    assert ms >= 0, er.last_error(lib)
+    samples = er.samples_np(lib)
+    col = er.colours_np(lib)
+    ref = oracle_c1.render(config1["eyes"][0], er.pose_np(lib), spp=32, want_samples=True)
+    assert np.array_equal(samples.view(np.uint32), ref["samples"].view(np.uint32))
+    assert np.array_equal(col, ref["colour_gpu_order"][0])
+    # the reference's own (sequential) summation order: within 1e-5 relative
+    assert rel_close(col, ref["colour_seq"][0], 1e-5)
+
+
+def test_frame_buffer_make_color(lib, config1, oracle_c1):
+    setup_eye(lib, config1, spp=16)
+    er.setRenderSize(lib, 1000, 1)
+    lib.renderFrame()
+    frame = np.array(lib.getFramePointer(), copy=True)
+    col = er.colours_np(lib)
+    assert np.array_equal(frame[0], oracle.make_color(col))
+    er.setRenderSize(lib, 400, 400)
+
+
+def test_frame_counter_and_restart(lib, config1, oracle_c1):
+    omm = config1["eyes"][0][:256]
+    setup_eye(lib, config1, omm=omm, spp=8)
+    pose = er.pose_np(lib)
+    lib.renderFrame()
+    lib.renderFrame()
+    assert lib.getCurrentEyeFrameIndex() == 2
+    want = oracle_c1.render(omm, pose, spp=8, frame=1)["colour_gpu_order"][0]
+    assert np.array_equal(er.colours_np(lib), want)
+    lib.setCurrentEyeSamplesPerOmmatidium(8)   # restarts the streams, as the reference re-seeds
+    assert lib.getCurrentEyeFrameIndex() == 0
+    lib.renderFrame()
+    want0 = oracle_c1.render(omm, pose, spp=8, frame=0)["colour_gpu_order"][0]
+    assert np.array_equal(er.colours_np(lib), want0)
+
+
+def test_uneven_sample_counts(lib, config1, oracle_c1):
+    omm = config1["eyes"][0][:500]
+    counts = scenes.log_uniform_counts(500, 1, 300, seed=9)
+    counts[7] = 0
+    counts[8] = 64
+    counts[9] = 65
+    setup_eye(lib, config1, omm=omm, spp=4)
+    er.set_sample_counts(lib, counts)
+    assert lib.getCurrentEyeTotalSamples() == int(counts.sum())
+    assert lib.renderFrame() >= 0, er.last_error(lib)
+    ref = oracle_c1.render(omm, er.pose_np(lib), counts=counts, want_samples=True)
+    assert np.array_equal(er.samples_np(lib).view(np.uint32), ref["samples"].view(np.uint32))
+    col = er.colours_np(lib)
+    assert np.array_equal(col, ref["colour_gpu_order"][0])
+    assert np.all(col[7] == 0.0)
+    er.set_sample_counts(lib, None)
+
+
+def test_pose_batch(lib, config1, oracle_c1):
+    omm = config1["eyes"][0][:300]
+    setup_eye(lib, config1, omm=omm, spp=16)
+    poses6 = scenes.batch_poses(config1["scene"], 12, seed=4, extent=10.0)
+    ms, rgb = er.render_batch_np(lib, poses6)
+    assert ms >= 0
+    poses12 = np.stack([oracle.pose12(p) for p in poses6])
+    ref = oracle_c1.render(omm, poses12, spp=16, frame=lib.getCurrentEyeFrameIndex())
+    assert np.array_equal(rgb, ref["colour_gpu_order"])
+
+
+def test_spp_edge_values(lib, config1, oracle_c1):
+    omm = config1["eyes"][0][:64]
+    for spp in (1, 63, 64, 65, 129):
+        setup_eye(lib, config1, omm=omm, spp=spp)
+        lib.renderFrame()
+        want = oracle_c1.render(omm, er.pose_np(lib), spp=spp)["colour_gpu_order"][0]
+        assert np.array_equal(er.colours_np(lib), want), spp
+    setup_eye(lib, config1, omm=omm, spp=0)
+    assert lib.getCurrentEyeSamplesPerOmmatidium() == 1
+
+
+def test_degenerate_axes(lib, config1, oracle_c1):
+    # axis straight up (perp falls back to +z) and axes with x == z (the
+    # reference's component-sum test also falls back there)
+    omm = np.zeros((4, 8), np.float32)
+    omm[0, 3:6] = (0, 1, 0)
+    omm[1, 3:6] = (0, -1, 0)
+    omm[2, 3:6] = (0.6, 0.52915025, 0.6)
+    omm[3, 3:6] = (-0.5, -0.70710677, -0.5)
+    omm[:, 6] = 0.05
+    setup_eye(lib, config1, omm=omm, spp=64)
+    rays = er.rays_np(lib)
+    want = oracle.generate_rays(omm, er.pose_np(lib), spp=64, eye=lib.getCurrentEyeIndex())
+    assert np.array_equal(rays.view(np.uint32), want.view(np.uint32))
+    lib.renderFrame()
+    ref = oracle_c1.render(omm, er.pose_np(lib), spp=64)["colour_gpu_order"][0]
+    assert np.array_equal(er.colours_np(lib), ref)
+
+
+def test_empty_and_tiny_scenes(lib, tmp_path):
+    rng = np.random.default_rng(3)
+    eye = scenes.fibonacci_eye(200, 0.001, 0.05)
+    # a scene with no triangles: every sample is sky
+    empty = scenes.Scene([], eyes=[scenes.Eye("eye", eye)])
+    path = scenes.write_gltf(tmp_path, "empty", empty)
+    load(lib, path)
+    lib.setCurrentEyeSamplesPerOmmatidium(8)
+    lib.setCameraPose(0, 0, 0, 0, 0, 0)
+    assert lib.renderFrame() >= 0, er.last_error(lib)
+    o = oracle.OracleScene(np.zeros((0, 9), np.float32), np.zeros((0, 9), np.float32), np.zeros(0, np.uint32), 1)
+    want = o.render(eye, er.pose_np(lib), spp=8)["colour_gpu_order"][0]
+    assert np.array_equal(er.colours_np(lib), want)
+    # a single triangle and a two-triangle scene (below the leaf size)
+    for ntri in (1, 2, 5):
+        pos = rng.uniform(-1, 1, size=(3 * ntri, 3)).astype(np.float32) + np.float32([0, 0, 2])
+        idx = np.arange(3 * ntri, dtype=np.uint32).reshape(-1, 3)
+        col = np.concatenate([rng.uniform(0, 1, (3 * ntri, 3)), np.ones((3 * ntri, 1))], 1).astype(np.float32)
+        sc = scenes.Scene([scenes.Mesh("t", pos, idx, col)], eyes=[scenes.Eye("eye", eye)])
+        path = scenes.write_gltf(tmp_path, f"tri{ntri}", sc)
+        load(lib, path)
+        lib.setCurrentEyeSamplesPerOmmatidium(16)
+        lib.setCameraPose(0, 0, 0, 0, 0, 0)
+        lib.renderFrame()
+        o = oracle.OracleScene.from_scene(sc)
+        assert np.array_equal(er.bvh_nodes_np(lib), o.nodes())
+        want = o.render(eye, er.pose_np(lib), spp=16)["colour_gpu_order"][0]
+        assert np.array_equal(er.colours_np(lib), want), ntri
