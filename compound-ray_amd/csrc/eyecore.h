@@ -169,6 +169,9 @@ struct RayPre {
   float tmin, tmax;
   // safe reciprocal direction for the slab test
   float ix, iy, iz;
+  // bit a set when direction component a is (numerically) zero: the ray runs
+  // parallel to that slab and is inside it iff lo <= origin <= hi
+  int par;
 };
 
 EYE_HD float safe_rcp(float d) {
@@ -196,6 +199,7 @@ EYE_HD RayPre precompute_ray(const Ray& r) {
   p.sz = 1.0f / dkz;
   p.tmin = r.tmin; p.tmax = r.tmax;
   p.ix = safe_rcp(r.dx); p.iy = safe_rcp(r.dy); p.iz = safe_rcp(r.dz);
+  p.par = (fabsf(r.dx) < 1e-30f ? 1 : 0) | (fabsf(r.dy) < 1e-30f ? 2 : 0) | (fabsf(r.dz) < 1e-30f ? 4 : 0);
   return p;
 }
 
@@ -234,12 +238,32 @@ EYE_HD bool intersect_triangle(const RayPre& r, V3 v0, V3 v1, V3 v2, float* t_ou
 // distance is widened by 1 + 2*gamma(3) so that rounding never culls a box
 // that holds a hit.  Returns the entry distance, or +inf for a miss.
 constexpr float kRobustFar = 1.00000048f;  // 1 + 4 ulp >= 1 + 2*gamma(3)
+// Boxes are culled against the current closest hit only when they start
+// beyond it by more than this relative slack: the box entry and the triangle
+// distance are rounded differently, and the (t, id) tie-break needs every
+// triangle at the closest distance to be tested, including at shared vertices.
+constexpr float kCullSlack = 1.0000152587890625f;  // 1 + 2^-16
 
+// Slab entry/exit along one axis; a parallel ray gets (-inf, inf) inside the
+// slab (boundaries included) and an empty interval outside it.
+template <bool kParallel>
+EYE_HD void slab_axis(float lo, float hi, float o, float inv, bool par, float* t0, float* t1) {
+  if (kParallel && par) {
+    *t0 = lo <= o ? -INFINITY : INFINITY;
+    *t1 = hi >= o ? INFINITY : -INFINITY;
+  } else {
+    *t0 = (lo - o) * inv;
+    *t1 = (hi - o) * inv;
+  }
+}
+
+template <bool kParallel = false>
 EYE_HD float slab_enter(const RayPre& r, float lox, float hix, float loy, float hiy,
                         float loz, float hiz, float tfar_limit) {
-  const float tx0 = (lox - r.ox) * r.ix, tx1 = (hix - r.ox) * r.ix;
-  const float ty0 = (loy - r.oy) * r.iy, ty1 = (hiy - r.oy) * r.iy;
-  const float tz0 = (loz - r.oz) * r.iz, tz1 = (hiz - r.oz) * r.iz;
+  float tx0, tx1, ty0, ty1, tz0, tz1;
+  slab_axis<kParallel>(lox, hix, r.ox, r.ix, (r.par & 1) != 0, &tx0, &tx1);
+  slab_axis<kParallel>(loy, hiy, r.oy, r.iy, (r.par & 2) != 0, &ty0, &ty1);
+  slab_axis<kParallel>(loz, hiz, r.oz, r.iz, (r.par & 4) != 0, &tz0, &tz1);
   const float tn = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), r.tmin));
   const float tf = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fmaxf(tz0, tz1)) * kRobustFar;
   return (tn <= fminf(tf, tfar_limit)) ? tn : INFINITY;

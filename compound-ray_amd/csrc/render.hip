@@ -63,10 +63,10 @@ __device__ __forceinline__ void test_leaf(const float4* __restrict__ tris, int32
 // Closest hit over the 4-wide BVH.  `stack` points at this lane's column of
 // an LDS array laid out [depth][blockDim.x].  The result is independent of
 // the traversal order: the (t, id) pair is minimised lexicographically.
-template <bool kStats = false>
-__device__ __forceinline__ void closest_hit(const SceneView& S, const RayPre& rp, uint32_t* stack,
-                                            int stride, Hit& h, uint32_t* nodes_visited = nullptr,
-                                            uint32_t* tris_tested = nullptr) {
+template <bool kStats, bool kParallel>
+__device__ __forceinline__ void closest_hit_impl(const SceneView& S, const RayPre& rp, uint32_t* stack,
+                                                 int stride, Hit& h, uint32_t* nodes_visited,
+                                                 uint32_t* tris_tested) {
   h.t = rp.tmax;
   h.b1 = h.b2 = 0.0f;
   h.k = 0xffffffffu;
@@ -80,10 +80,11 @@ __device__ __forceinline__ void closest_hit(const SceneView& S, const RayPre& rp
     const float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5];
     const int4 ch = reinterpret_cast<const int4*>(np)[6];
     float te[4];
-    te[0] = slab_enter(rp, lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, h.t);
-    te[1] = slab_enter(rp, lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, h.t);
-    te[2] = slab_enter(rp, lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, h.t);
-    te[3] = slab_enter(rp, lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, h.t);
+    const float limit = h.t * kCullSlack;
+    te[0] = slab_enter<kParallel>(rp, lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, limit);
+    te[1] = slab_enter<kParallel>(rp, lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, limit);
+    te[2] = slab_enter<kParallel>(rp, lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, limit);
+    te[3] = slab_enter<kParallel>(rp, lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, limit);
     const int32_t cc[4] = {ch.x, ch.y, ch.z, ch.w};
     int ni = 0;
     int ic[4];
@@ -126,6 +127,18 @@ __device__ __forceinline__ void closest_hit(const SceneView& S, const RayPre& rp
     }
     node = ic[0];
   }
+}
+
+// Rays with an exactly axis-parallel direction take the variant with the
+// parallel-slab rule; all others the plain slab test.
+template <bool kStats = false>
+__device__ __forceinline__ void closest_hit(const SceneView& S, const RayPre& rp, uint32_t* stack, int stride,
+                                            Hit& h, uint32_t* nodes_visited = nullptr,
+                                            uint32_t* tris_tested = nullptr) {
+  if (rp.par != 0)
+    closest_hit_impl<kStats, true>(S, rp, stack, stride, h, nodes_visited, tris_tested);
+  else
+    closest_hit_impl<kStats, false>(S, rp, stack, stride, h, nodes_visited, tris_tested);
 }
 
 __device__ __forceinline__ V3 shade(const SceneView& S, const Hit& h, float dx, float dy, float dz) {

@@ -237,6 +237,7 @@ struct Pre {
   int kx, ky, kz;
   float sx, sy, sz, tmin, tmax;
   float inv[3];
+  bool par[3];
 };
 
 Pre prepare(const RayO& r) {
@@ -256,7 +257,8 @@ Pre prepare(const RayO& r) {
   p.tmax = r.tmax;
   for (int a = 0; a < 3; ++a) {
     float d = r.d[a];
-    if (std::fabs(d) < 1e-30f) d = d < 0.0f ? -1e-30f : 1e-30f;
+    p.par[a] = std::fabs(d) < 1e-30f;
+    if (p.par[a]) d = d < 0.0f ? -1e-30f : 1e-30f;
     p.inv[a] = 1.0f / d;
   }
   return p;
@@ -530,10 +532,22 @@ HitO trace_brute(const Scene& S, const RayO& r) {
   return h;
 }
 
+// One slab; an axis-parallel ray (|d| < 1e-30) is inside it iff lo <= o <= hi.
+void slab_axis(const Pre& r, int a, float lo, float hi, float* t0, float* t1) {
+  if (r.par[a]) {
+    *t0 = lo <= r.o[a] ? -INFINITY : INFINITY;
+    *t1 = hi >= r.o[a] ? INFINITY : -INFINITY;
+  } else {
+    *t0 = (lo - r.o[a]) * r.inv[a];
+    *t1 = (hi - r.o[a]) * r.inv[a];
+  }
+}
+
 float slab(const Pre& r, const Node4& nd, int s, float limit) {
-  const float x0 = (nd.lox[s] - r.o[0]) * r.inv[0], x1 = (nd.hix[s] - r.o[0]) * r.inv[0];
-  const float y0 = (nd.loy[s] - r.o[1]) * r.inv[1], y1 = (nd.hiy[s] - r.o[1]) * r.inv[1];
-  const float z0 = (nd.loz[s] - r.o[2]) * r.inv[2], z1 = (nd.hiz[s] - r.o[2]) * r.inv[2];
+  float x0, x1, y0, y1, z0, z1;
+  slab_axis(r, 0, nd.lox[s], nd.hix[s], &x0, &x1);
+  slab_axis(r, 1, nd.loy[s], nd.hiy[s], &y0, &y1);
+  slab_axis(r, 2, nd.loz[s], nd.hiz[s], &z0, &z1);
   const float tn = std::fmax(std::fmax(std::fmin(x0, x1), std::fmin(y0, y1)), std::fmax(std::fmin(z0, z1), r.tmin));
   const float tf = std::fmin(std::fmin(std::fmax(x0, x1), std::fmax(y0, y1)), std::fmax(z0, z1)) * 1.00000048f;
   return tn <= std::fmin(tf, limit) ? tn : INFINITY;
@@ -552,7 +566,8 @@ HitO trace_bvh(const Scene& S, const RayO& r, uint64_t* nodes_visited, uint64_t*
     const Node4& nd = S.nodes[(size_t)node];
     ++*nodes_visited;
     float te[4];
-    for (int s = 0; s < 4; ++s) te[s] = slab(p, nd, s, h.t);
+    // cull only boxes entered beyond the closest hit by more than 2^-16 relative
+    for (int s = 0; s < 4; ++s) te[s] = slab(p, nd, s, h.t * 1.0000152587890625f);
     int ni = 0, ic[4];
     float it[4];
     for (int s = 0; s < 4; ++s) {
