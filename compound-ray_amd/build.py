@@ -27,6 +27,7 @@ SOURCES = [
     "lbvh.hip",
     "render.hip",
     "batch.cpp",
+    "mlp.hip",
     "eye_renderer.cpp",
 ]
 

@@ -18,6 +18,7 @@
 #include "../../include/eye_renderer.h"
 #include "batch.h"
 #include "device_scene.h"
+#include "mlp.h"
 #include "render.h"
 #include "scene_loader.h"
 
@@ -88,6 +89,7 @@ struct State {
   double timings[4] = {0, 0, 0, 0};   // trace, reduce, projection, total GPU ms of the last render
   unsigned long long* d_stats = nullptr;
   BatchRenderer batch;
+  Mlp mlp;
   std::string name_buf;
 };
 
@@ -283,6 +285,22 @@ void sync_frame_host() {
   hip_check(hipMemcpy(s.frame_host.data(), s.d_frame, n * 4, hipMemcpyDeviceToHost), "frame readback");
   s.frame_host_stale = false;
 }
+
+// Scoped device copies of host arrays (freed on scope exit).
+struct DeviceTemp {
+  std::vector<void*> ptrs;
+  ~DeviceTemp() {
+    for (void* p : ptrs) (void)hipFree(p);
+  }
+  template <typename T>
+  T* upload(const T* host, size_t n) {
+    T* d = nullptr;
+    hip_check(hipMalloc(&d, sizeof(T) * (n ? n : 1)), "hipMalloc");
+    ptrs.push_back(d);
+    if (host && n) hip_check(hipMemcpy(d, host, sizeof(T) * n, hipMemcpyHostToDevice), "upload");
+    return d;
+  }
+};
 
 #define ABI_GUARD_BEGIN try {
 #define ABI_GUARD_END(ret)                      \
@@ -844,6 +862,88 @@ double renderPoseBatch(const float* poses, size_t poseCount, float* rgbOut) {
   if (!s.batch.render(s.scene, L, ps, rgbOut, s.stream, &ms, &err)) throw std::runtime_error(err);
   return ms;
   ABI_GUARD_END(return -1.0)
+}
+
+size_t mlpParamCount(void) { return (size_t)kMlpParams; }
+
+int mlpInit(uint32_t seed) {
+  ABI_GUARD_BEGIN
+  ensure_hip();
+  std::string err;
+  if (!st().mlp.init(seed, st().stream, &err)) throw std::runtime_error(err);
+  return 0;
+  ABI_GUARD_END(return -1)
+}
+
+int mlpSetParams(const float* params, size_t count) {
+  ABI_GUARD_BEGIN
+  if (!params || count != (size_t)kMlpParams) throw std::runtime_error("mlpSetParams: need mlpParamCount() floats");
+  ensure_hip();
+  std::string err;
+  if (!st().mlp.set_params(params, st().stream, &err)) throw std::runtime_error(err);
+  return 0;
+  ABI_GUARD_END(return -1)
+}
+
+int mlpGetParams(float* params, size_t count) {
+  ABI_GUARD_BEGIN
+  if (!params || count != (size_t)kMlpParams) throw std::runtime_error("mlpGetParams: need mlpParamCount() floats");
+  ensure_hip();
+  std::string err;
+  if (!st().mlp.get_params(params, st().stream, &err)) throw std::runtime_error(err);
+  return 0;
+  ABI_GUARD_END(return -1)
+}
+
+int mlpPredict(const float* features, size_t rows, float* out) {
+  ABI_GUARD_BEGIN
+  if (rows == 0) return 0;
+  if (!features || !out) throw std::runtime_error("mlpPredict: null argument");
+  ensure_hip();
+  DeviceTemp tmp;
+  const float* dx = tmp.upload(features, rows * kMlpIn);
+  float* dy = tmp.upload<float>(nullptr, rows);
+  std::string err;
+  if (!st().mlp.predict(dx, (int64_t)rows, dy, st().stream, &err)) throw std::runtime_error(err);
+  hip_check(hipStreamSynchronize(st().stream), "mlp");
+  hip_check(hipMemcpy(out, dy, sizeof(float) * rows, hipMemcpyDeviceToHost), "readback");
+  return 0;
+  ABI_GUARD_END(return -1)
+}
+
+int mlpLossGrad(const float* features, const float* targets, size_t rows, float* loss, float* grad) {
+  ABI_GUARD_BEGIN
+  if (!features || !targets || rows == 0) throw std::runtime_error("mlpLossGrad: empty input");
+  ensure_hip();
+  DeviceTemp tmp;
+  const float* dx = tmp.upload(features, rows * kMlpIn);
+  const float* dt = tmp.upload(targets, rows);
+  float* dl = tmp.upload<float>(nullptr, 1);
+  float* dg = tmp.upload<float>(nullptr, kMlpParams);
+  std::string err;
+  if (!st().mlp.loss_grad(dx, dt, (int64_t)rows, dl, dg, st().stream, &err)) throw std::runtime_error(err);
+  hip_check(hipStreamSynchronize(st().stream), "mlp");
+  if (loss) hip_check(hipMemcpy(loss, dl, sizeof(float), hipMemcpyDeviceToHost), "readback");
+  if (grad) hip_check(hipMemcpy(grad, dg, sizeof(float) * kMlpParams, hipMemcpyDeviceToHost), "readback");
+  return 0;
+  ABI_GUARD_END(return -1)
+}
+
+int mlpTrain(const float* features, const float* targets, size_t rows, int epochs, int batch, float lr,
+             uint32_t seed, float* epochLosses) {
+  ABI_GUARD_BEGIN
+  if (!features || !targets || rows == 0) throw std::runtime_error("mlpTrain: empty input");
+  ensure_hip();
+  DeviceTemp tmp;
+  const float* dx = tmp.upload(features, rows * kMlpIn);
+  const float* dt = tmp.upload(targets, rows);
+  std::vector<float> losses;
+  std::string err;
+  if (!st().mlp.train(dx, dt, (int64_t)rows, epochs, batch, lr, seed, &losses, st().stream, &err))
+    throw std::runtime_error(err);
+  if (epochLosses) std::copy(losses.begin(), losses.end(), epochLosses);
+  return 0;
+  ABI_GUARD_END(return -1)
 }
 
 int getSceneBVHInfo(int64_t* info) {

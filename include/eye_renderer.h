@@ -160,6 +160,20 @@ int traceRays(const float* rays, size_t count, int32_t* primIds, float* tbb);
  * Pose k uses counter word 3 = k.  Returns GPU milliseconds, or -1 on error. */
 double renderPoseBatch(const float* poses, size_t poseCount, float* rgbOut);
 
+/* ---- Learned sampler MLP (12 -> 64 -> 64 -> 64 -> 1, BF16 on MFMA) ----
+ * Parameters are FP32 in the order W1[12][64], b1[64], W2[64][64], b2[64],
+ * W3[64][64], b3[64], W4[64], b4.  Host arrays; all return 0 or -1. */
+size_t mlpParamCount(void);
+int mlpInit(uint32_t seed);
+int mlpSetParams(const float* params, size_t count);
+int mlpGetParams(float* params, size_t count);
+int mlpPredict(const float* features, size_t rows, float* out);
+/* Mean squared error over the rows and its gradient (FP32, mlpParamCount). */
+int mlpLossGrad(const float* features, const float* targets, size_t rows, float* loss, float* grad);
+/* Adam minibatch training; epochLosses (nullable) receives one mean loss per epoch. */
+int mlpTrain(const float* features, const float* targets, size_t rows, int epochs, int batch, float lr,
+             uint32_t seed, float* epochLosses);
+
 /* Acceleration-structure inspection: info[0..7] = triangle count, node count,
  * stack bound, levels, leaf size, build ms * 1000, 0, 0.  Returns 0 or -1. */
 int getSceneBVHInfo(int64_t* info);

@@ -169,3 +169,19 @@ def generate_rays(omm, poses12, spp: int = 32, counts=None, seed: int = 42, eye:
     out = np.zeros((P.shape[0] * per_pose, 8), dtype=np.float32)
     lib().oracle_generate_rays(_p(o), o.shape[0], _p(c), spp, _p(P), P.shape[0], seed, eye, frame, _p(out))
     return out
+
+
+def mlp_loss_grad(params, X, T=None, mode: int = 0):
+    """FP64 MLP (mode 0 exact, mode 1 with the GPU's BF16 rounding points).
+    Returns (y, grad or None, loss)."""
+    p = np.ascontiguousarray(params, dtype=np.float32)
+    x = np.ascontiguousarray(X, dtype=np.float32).reshape(-1, 12)
+    n = x.shape[0]
+    t = None if T is None else np.ascontiguousarray(T, dtype=np.float32)
+    y = np.zeros(n, dtype=np.float64)
+    grad = np.zeros(p.size, dtype=np.float64) if t is not None else None
+    loss = np.zeros(1, dtype=np.float64)
+    L = lib()
+    L.oracle_mlp_loss_grad.argtypes = [c_void_p, c_void_p, c_void_p, c_int64, c_int, c_void_p, c_void_p, c_void_p]
+    L.oracle_mlp_loss_grad(_p(p), _p(x), _p(t), n, mode, _p(y), _p(grad), _p(loss))
+    return y, grad, float(loss[0])

@@ -841,6 +841,93 @@ void oracle_pose(const float* pos, const float* rot, float* out) {
   for (int a = 0; a < 3; ++a) { out[3 + 3 * a] = ax[a].x; out[4 + 3 * a] = ax[a].y; out[5 + 3 * a] = ax[a].z; }
 }
 
+// ------------------------------------------------------ learned sampler ----
+// FP64 12-64-64-64-1 ReLU MLP with the GPU kernels' parameter order
+// (W1[12][64], b1, W2[64][64], b2, W3[64][64], b3, W4[64], b4; W[in][out]).
+// mode 0: exact FP64.  mode 1: BF16 rounding (nearest even) of the inputs,
+// weights, hidden activations and pre-activation gradients where the GPU
+// feeds them to MFMA, FP64 sums elsewhere.
+static double bf16r(double x) {
+  const float f = (float)x;
+  uint32_t u = bits(f);
+  u = (u + 0x7fffu + ((u >> 16) & 1u)) & 0xffff0000u;
+  return (double)from_bits(u);
+}
+
+void oracle_mlp_loss_grad(const float* params, const float* X, const float* T, int64_t n, int mode, double* y_out,
+                          double* grad, double* loss_out) {
+  const int H = 64, IN = 12;
+  auto R = [&](double v) { return mode ? bf16r(v) : v; };
+  const float* W1 = params;
+  const float* b1 = W1 + IN * H;
+  const float* W2 = b1 + H;
+  const float* b2 = W2 + H * H;
+  const float* W3 = b2 + H;
+  const float* b3 = W3 + H * H;
+  const float* W4 = b3 + H;
+  const float b4 = W4[H];
+  std::vector<double> w1(IN * H), w2(H * H), w3(H * H), w4(H);
+  for (int k = 0; k < IN * H; ++k) w1[k] = R(W1[k]);
+  for (int k = 0; k < H * H; ++k) { w2[k] = R(W2[k]); w3[k] = R(W3[k]); }
+  for (int k = 0; k < H; ++k) w4[k] = R(W4[k]);
+  const int P = IN * H + H + H * H + H + H * H + H + H + 1;
+  if (grad) std::fill(grad, grad + P, 0.0);
+  double* gW1 = grad;
+  double* gb1 = grad ? gW1 + IN * H : nullptr;
+  double* gW2 = grad ? gb1 + H : nullptr;
+  double* gb2 = grad ? gW2 + H * H : nullptr;
+  double* gW3 = grad ? gb2 + H : nullptr;
+  double* gb3 = grad ? gW3 + H * H : nullptr;
+  double* gW4 = grad ? gb3 + H : nullptr;
+  double loss = 0.0;
+  std::vector<double> x(IN), h1(H), h2(H), h3(H), d3(H), d2(H), d1(H);
+  for (int64_t s = 0; s < n; ++s) {
+    for (int i = 0; i < IN; ++i) x[i] = R(X[s * IN + i]);
+    auto layer = [&](const std::vector<double>& in, int nin, const std::vector<double>& w, const float* b,
+                     std::vector<double>& out) {
+      for (int o = 0; o < H; ++o) {
+        double z = 0.0;
+        for (int i = 0; i < nin; ++i) z += in[i] * w[i * H + o];
+        z = mode ? (double)((float)z + b[o]) : z + b[o];
+        out[o] = R(z > 0.0 ? z : 0.0);
+      }
+    };
+    layer(x, IN, w1, b1, h1);
+    layer(h1, H, w2, b2, h2);
+    layer(h2, H, w3, b3, h3);
+    double y = b4;
+    for (int f = 0; f < H; ++f) y += h3[f] * w4[f];
+    if (y_out) y_out[s] = y;
+    if (!T) continue;
+    const double err = y - T[s];
+    loss += err * err;
+    if (!grad) continue;
+    const double dy = 2.0 * err / (double)n;
+    gW4[H] += dy;
+    for (int f = 0; f < H; ++f) {
+      gW4[f] += dy * h3[f];
+      d3[f] = h3[f] > 0.0 ? w4[f] * dy : 0.0;
+      gb3[f] += d3[f];
+    }
+    auto back = [&](const std::vector<double>& d, const std::vector<double>& hin, int nin,
+                    const std::vector<double>& w, double* gW, std::vector<double>* dprev, double* gbprev) {
+      for (int i = 0; i < nin; ++i)
+        for (int o = 0; o < H; ++o) gW[i * H + o] += hin[i] * R(d[o]);
+      if (!dprev) return;
+      for (int i = 0; i < nin; ++i) {
+        double acc = 0.0;
+        for (int o = 0; o < H; ++o) acc += w[i * H + o] * R(d[o]);
+        (*dprev)[i] = hin[i] > 0.0 ? acc : 0.0;
+        gbprev[i] += (*dprev)[i];
+      }
+    };
+    back(d3, h2, H, w3, gW3, &d2, gb2);
+    back(d2, h1, H, w2, gW2, &d1, gb1);
+    back(d1, x, IN, w1, gW1, nullptr, nullptr);
+  }
+  if (loss_out) *loss_out = T ? loss / (double)n : 0.0;
+}
+
 // make_color, shaders.cu:180-189: 4 bytes per colour.
 void oracle_make_color(const float* rgb, int64_t n, uint8_t* out) {
   const float g = (float)(1.0 / (double)2.2f);

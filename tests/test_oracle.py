@@ -381,3 +381,53 @@ def test_monte_carlo_convergence(config1, oracle_c1):
         errs.append(float(np.sqrt(np.mean((c - ref) ** 2))))
     r1, r2 = errs[0] / errs[1], errs[1] / errs[2]
     assert 1.5 < r1 < 2.7 and 1.5 < r2 < 2.7, errs
+
+
+# ------------------------------------------------------------------ MLP ----
+
+def mlp_params(seed=0, scale=1.0):
+    rng = np.random.default_rng(seed)
+    sizes = [(12 * 64, 12), (64, 0), (64 * 64, 64), (64, 0), (64 * 64, 64), (64, 0), (64, 64), (1, 0)]
+    parts = []
+    for n, fan in sizes:
+        lim = math.sqrt(6.0 / fan) if fan else 0.1
+        parts.append(rng.uniform(-lim, lim, n) * scale)
+    return np.concatenate(parts).astype(np.float32)
+
+
+def test_mlp_backprop_matches_finite_differences():
+    """The FP64 oracle MLP's analytic gradient equals central differences
+    (which pins the backward pass the GPU gradients are compared with)."""
+    rng = np.random.default_rng(1)
+    p = mlp_params(1)
+    x = rng.normal(size=(40, 12)).astype(np.float32)
+    t = rng.normal(size=40).astype(np.float32)
+    _, g, loss = oracle.mlp_loss_grad(p, x, t, mode=0)
+    assert loss > 0
+    idx = np.concatenate([rng.choice(768, 12, replace=False), 768 + rng.choice(64, 6, replace=False),
+                          832 + rng.choice(4096, 12, replace=False), 4992 + rng.choice(4096, 12, replace=False),
+                          9088 + rng.choice(64, 6, replace=False), 9152 + np.arange(0, 64, 9), [9216]])
+    p64 = p.astype(np.float64)
+    for k in idx:
+        # small steps keep the ReLU kinks out of the difference quotient
+        h = 1e-5 * max(1.0, abs(p64[k]))
+        # parameters are float32 at the boundary: use perturbations exactly representable
+        hp = np.float32(p64[k] + h) - np.float32(p64[k])
+        pp, pm = p.copy(), p.copy()
+        pp[k] += hp
+        pm[k] -= hp
+        lp = oracle.mlp_loss_grad(pp, x, t, mode=0)[2]
+        lm = oracle.mlp_loss_grad(pm, x, t, mode=0)[2]
+        fd = (lp - lm) / (2 * float(hp))
+        assert abs(fd - g[k]) <= 2e-3 * max(1e-3, abs(g[k])) + 1e-6, (k, fd, g[k])
+
+
+def test_mlp_bf16_mode_close_to_exact():
+    rng = np.random.default_rng(2)
+    p = mlp_params(2)
+    x = rng.normal(size=(200, 12)).astype(np.float32)
+    t = rng.normal(size=200).astype(np.float32)
+    y0, g0, _ = oracle.mlp_loss_grad(p, x, t, mode=0)
+    y1, g1, _ = oracle.mlp_loss_grad(p, x, t, mode=1)
+    assert np.max(np.abs(y1 - y0)) < 0.05 * np.max(np.abs(y0))
+    assert np.linalg.norm(g1 - g0) < 0.05 * np.linalg.norm(g0)
