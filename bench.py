@@ -51,10 +51,17 @@ def cpu_baseline(cfg: dict, spp: int, pose12: np.ndarray, target_s: float = 10.0
     osc.render(omm[:m], pose12, spp=spp, threads=threads)
     dt = max(time.perf_counter() - t0, 1e-3)
     m = int(min(len(omm), max(64, m * target_s / dt)))
+    # whole frames of the first m ommatidia (consecutive frame indices) until
+    # about target_s seconds of CPU work
+    frames = 0
     t0 = time.perf_counter()
-    osc.render(omm[:m], pose12, spp=spp, threads=threads)
-    dt = time.perf_counter() - t0
-    rays = m * spp
+    while True:
+        osc.render(omm[:m], pose12, spp=spp, frame=frames, threads=threads)
+        frames += 1
+        dt = time.perf_counter() - t0
+        if dt >= target_s or frames >= 1000:
+            break
+    rays = m * spp * frames
     rays_per_s = rays / dt
     eye_frame_rays = len(omm) * spp
     cpu_model = platform.processor() or "unknown"
@@ -70,9 +77,41 @@ def cpu_baseline(cfg: dict, spp: int, pose12: np.ndarray, target_s: float = 10.0
         "unit": "eye-frames/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"eye 0, first {m} of {len(omm)} ommatidia x {spp} spp ({rays} rays) in {dt:.2f} s; "
+        "sample": f"eye 0, first {m} of {len(omm)} ommatidia x {spp} spp x {frames} frames ({rays} rays) in "
+                  f"{dt:.2f} s; "
                   f"{rays_per_s / 1e6:.3f} Mrays/s; oracle BVH build {build_s:.1f} s excluded; CPU {cpu_model}",
         "grays_per_s": rays_per_s / 1e9,
+    }
+
+
+def bench_batched(lib, er, scenes, tmp: str, repeats: int = 3) -> dict:
+    """Batched configuration: 4,096 seeded poses of one 5,000-ommatidium eye
+    over the 19,602-triangle heightfield, per-ommatidium samples log-uniform in
+    [8, 256], one persistent graph-launched render per batch."""
+    cfg = scenes.build_config(3, os.path.join(tmp, "c3"))
+    lib.loadGlTFscene(str(cfg["gltf"]).encode())
+    if er.last_error(lib):
+        raise RuntimeError(er.last_error(lib))
+    er.gotoFirstCompoundEye(lib)
+    er.set_sample_counts(lib, cfg["counts"])
+    poses = cfg["poses"]
+    rays = int(cfg["counts"].sum()) * len(poses)
+    er.render_batch_np(lib, poses, want_rgb=False)  # warm-up (captures the graph)
+    times = []
+    t0 = time.perf_counter()
+    for _ in range(repeats):
+        ms, _ = er.render_batch_np(lib, poses, want_rgb=False)
+        times.append(ms)
+    wall = (time.perf_counter() - t0) / repeats
+    gpu_ms = float(np.mean(times))
+    return {
+        "workload": "4,096 poses x 5,000 ommatidia, spp log-uniform [8, 256] "
+                    f"({rays} rays/batch), 19,602-triangle heightfield",
+        "eye_frames_per_s": len(poses) / (wall),
+        "grays_per_s": rays / wall / 1e9,
+        "gpu_ms_per_batch": gpu_ms,
+        "wall_ms_per_batch": wall * 1e3,
+        "eye_frames_per_s_gpu": len(poses) / (gpu_ms * 1e-3),
     }
 
 
@@ -82,6 +121,11 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--profile", action="store_true",
+                    help="render loop only (no 4096-spp error check, no CPU baseline): for profiler runs")
+    ap.add_argument("--skip-batched", action="store_true", help="skip the 4,096-pose batched configuration")
+    ap.add_argument("--copy-probe", type=float, default=0.0,
+                    help="also run a plain copy kernel of this many MiB (profiler bandwidth reference)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this renderer runs on one GPU; --gpus above 1 is not supported "
@@ -152,16 +196,22 @@ def main() -> int:
     bytes_launch = (stats["nodes_visited"] * 128 + stats["tris_tested"] * 48 + stats["hits"] * 48
                     + rays_per_step * (32.0 / spp + 16.0)) / len(eyes)
     achieved = bytes_launch / (trace_avg_ms * 1e-3) / 1e9
-    # RMS colour error of this render against a 4096-spp render of the same eye
-    lib.gotoCamera(eyes[0])
-    col128 = er.colours_np(lib)
-    lib.setCurrentEyeSamplesPerOmmatidium(4096)
-    lib.setCurrentEyeFrameIndex(1 << 20)
-    lib.renderFrame()
-    col4096 = er.colours_np(lib)
-    lib.setCurrentEyeSamplesPerOmmatidium(spp)
-    rms = float(np.sqrt(np.mean((col128.astype(np.float64) - col4096) ** 2)))
-    rms_rel = rms / float(np.sqrt(np.mean(col4096.astype(np.float64) ** 2)))
+    if args.copy_probe > 0:
+        nbytes = int(args.copy_probe * 2 ** 20)
+        cms = lib.runCopyProbe(nbytes, 10)
+        log(f"copy probe: {nbytes} B in {cms:.4f} ms = {2 * nbytes / cms / 1e6:.1f} GB/s (read + write)")
+    rms = rms_rel = None
+    if not args.profile:
+        # RMS colour error of this render against a 4096-spp render of the same eye
+        lib.gotoCamera(eyes[0])
+        col128 = er.colours_np(lib)
+        lib.setCurrentEyeSamplesPerOmmatidium(4096)
+        lib.setCurrentEyeFrameIndex(1 << 20)
+        lib.renderFrame()
+        col4096 = er.colours_np(lib)
+        lib.setCurrentEyeSamplesPerOmmatidium(spp)
+        rms = float(np.sqrt(np.mean((col128.astype(np.float64) - col4096) ** 2)))
+        rms_rel = rms / float(np.sqrt(np.mean(col4096.astype(np.float64) ** 2)))
     out = {
         "metric": METRIC,
         "value": value,
@@ -190,12 +240,15 @@ def main() -> int:
                               "hit_fraction": stats["hits"] / rays_per_step},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_trace_samples", "bytes_per_launch": bytes_launch},
+                     "kernel": "k_render_units", "bytes_per_launch": bytes_launch},
     }
-    if not args.no_cpu_baseline:
+    if not (args.no_cpu_baseline or args.profile):
         lib.gotoCamera(eyes[0])
         pose12 = er.pose_np(lib)
         out["cpu_baseline"] = cpu_baseline(cfg, spp, pose12)
+    if not args.profile and not args.skip_batched:
+        log("batched configuration ...")
+        out["batched"] = bench_batched(lib, er, scenes, tmp)
     lib.stop()
     print(json.dumps(out))
     return 0

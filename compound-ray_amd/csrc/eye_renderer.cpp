@@ -39,6 +39,12 @@ struct EyeState {
   size_t d_omms_cap = 0;
   uint32_t* d_offsets = nullptr;
   size_t d_offsets_cap = 0;
+  uint32_t* d_counts = nullptr;          // per-ommatidium sample counts (when counts are set)
+  size_t d_counts_cap = 0;
+  uint64_t* d_sample_offsets = nullptr;  // first per-sample record of each ommatidium
+  size_t d_sample_offsets_cap = 0;
+  uint32_t* d_order = nullptr;           // processing order: ommatidia sorted by axis direction
+  size_t d_order_cap = 0;
   bool device_dirty = true;
   float4* d_samples = nullptr;
   size_t d_samples_cap = 0;
@@ -90,6 +96,10 @@ struct State {
   unsigned long long* d_stats = nullptr;
   BatchRenderer batch;
   Mlp mlp;
+  bool record_samples = false;   // keep per-sample records of renderFrame
+  uint32_t* d_queue = nullptr;   // work counter of the unit renderer
+  int units_per_grab = 4;
+  int unit_blocks = 0;           // 0: automatic
   std::string name_buf;
 };
 
@@ -122,6 +132,7 @@ void ensure_hip() {
   hip_check(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking), "hipStreamCreate");
   hip_check(hipMalloc(&s.d_pose, sizeof(Pose)), "hipMalloc pose");
   hip_check(hipMalloc(&s.d_stats, sizeof(unsigned long long) * 4), "hipMalloc stats");
+  hip_check(hipMalloc(&s.d_queue, sizeof(uint32_t) * 4), "hipMalloc queue");
   for (hipEvent_t& e : s.ev) hip_check(hipEventCreate(&e), "hipEventCreate");
   s.hip_ready = true;
 }
@@ -162,6 +173,9 @@ void free_scene() {
     if (e.d_offsets) (void)hipFree(e.d_offsets);
     if (e.d_samples) (void)hipFree(e.d_samples);
     if (e.d_colours) (void)hipFree(e.d_colours);
+    if (e.d_counts) (void)hipFree(e.d_counts);
+    if (e.d_sample_offsets) (void)hipFree(e.d_sample_offsets);
+    if (e.d_order) (void)hipFree(e.d_order);
   }
   s.eyes.clear();
   s.cams.clear();
@@ -177,11 +191,40 @@ void upload_eye(EyeState& e) {
     hip_check(hipMemcpyAsync(e.d_omms, e.omms.data(), sizeof(Ommatidium) * e.omms.size(), hipMemcpyHostToDevice,
                              s.stream),
               "upload ommatidia");
+  const size_t n = e.omms.size();
+  std::vector<uint64_t> first(n);
   if (!e.counts.empty()) {
     ensure_device(&e.d_offsets, &e.d_offsets_cap, e.offsets.size());
     hip_check(hipMemcpyAsync(e.d_offsets, e.offsets.data(), sizeof(uint32_t) * e.offsets.size(),
                              hipMemcpyHostToDevice, s.stream),
               "upload offsets");
+    ensure_device(&e.d_counts, &e.d_counts_cap, n);
+    hip_check(hipMemcpyAsync(e.d_counts, e.counts.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice, s.stream),
+              "upload counts");
+    for (size_t i = 0; i < n; ++i) first[i] = e.offsets[i];
+  } else {
+    for (size_t i = 0; i < n; ++i) first[i] = (uint64_t)i * (uint64_t)e.spp;
+  }
+  // Units are processed in the order of their axes' 3D Morton codes, so that
+  // waves running at the same time trace neighbouring view directions and
+  // share BVH nodes in cache.  Output positions are unchanged.
+  std::vector<std::pair<uint32_t, uint32_t>> key(n);
+  for (size_t i = 0; i < n; ++i) {
+    const V3 d = normalize(v3(e.omms[i].dx, e.omms[i].dy, e.omms[i].dz));
+    const float lo[3] = {-1.0f, -1.0f, -1.0f}, hi[3] = {1.0f, 1.0f, 1.0f};
+    const bool ok = d.x == d.x && d.y == d.y && d.z == d.z;
+    key[i] = {ok ? morton30(d.x, d.y, d.z, lo, hi) : 0u, (uint32_t)i};
+  }
+  std::sort(key.begin(), key.end());
+  std::vector<uint32_t> order(n);
+  for (size_t i = 0; i < n; ++i) order[i] = key[i].second;
+  ensure_device(&e.d_order, &e.d_order_cap, n);
+  ensure_device(&e.d_sample_offsets, &e.d_sample_offsets_cap, n);
+  if (n) {
+    hip_check(hipMemcpyAsync(e.d_order, order.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice, s.stream),
+              "upload order");
+    hip_check(hipMemcpyAsync(e.d_sample_offsets, first.data(), sizeof(uint64_t) * n, hipMemcpyHostToDevice, s.stream),
+              "upload sample offsets");
   }
   hip_check(hipStreamSynchronize(s.stream), "sync");
   e.device_dirty = false;
@@ -201,6 +244,25 @@ EyeLaunch launch_of(EyeState& e, const Pose* d_poses, int pose_count) {
   L.eye = e.eye_index;
   L.frame = e.frame;
   return L;
+}
+
+UnitLaunch unit_launch_of(EyeState& e, const Pose* d_poses, int pose_count) {
+  State& s = st();
+  UnitLaunch U;
+  U.omms = e.d_omms;
+  U.omm_count = (int)e.omms.size();
+  U.poses = d_poses;
+  U.pose_count = pose_count;
+  U.seed = s.seed;
+  U.eye = e.eye_index;
+  U.frame = e.frame;
+  U.spp = e.spp;
+  U.omm_counts = e.counts.empty() ? nullptr : e.d_counts;
+  U.colours = e.d_colours;
+  U.queue = s.d_queue;
+  U.units_per_grab = s.units_per_grab;
+  U.order = e.d_order;
+  return U;
 }
 
 void ensure_frame_buffer() {
@@ -226,7 +288,10 @@ double render_compound(EyeState& e, const Cam& c) {
   State& s = st();
   upload_eye(e);
   const uint64_t total = e.total_samples();
-  ensure_device(&e.d_samples, &e.d_samples_cap, (size_t)total);
+  // per-sample records are kept only when asked for (or for the raw projection)
+  const bool record = s.record_samples || e.shader == "raw_ommatidial_samples";
+  const bool units = trace_mode() == kTracePacket;
+  if (record || !units) ensure_device(&e.d_samples, &e.d_samples_cap, (size_t)total);
   ensure_device(&e.d_colours, &e.d_colours_cap, e.omms.size());
   ensure_frame_buffer();
   hip_check(hipMemcpyAsync(s.d_pose, &c.pose, sizeof(Pose), hipMemcpyHostToDevice, s.stream), "upload pose");
@@ -234,9 +299,19 @@ double render_compound(EyeState& e, const Cam& c) {
   std::string err;
   const auto t0 = std::chrono::steady_clock::now();
   hip_check(hipEventRecord(s.ev[0], s.stream), "event");
-  if (!launch_trace_samples(s.scene, L, e.d_samples, s.stream, &err)) throw std::runtime_error(err);
-  hip_check(hipEventRecord(s.ev[1], s.stream), "event");
-  if (!launch_reduce(L, e.d_samples, e.d_colours, s.stream, &err)) throw std::runtime_error(err);
+  if (units) {
+    UnitLaunch U = unit_launch_of(e, s.d_pose, 1);
+    if (record) {
+      U.samples = e.d_samples;
+      U.sample_offsets = e.d_sample_offsets;
+    }
+    if (!launch_render_units(s.scene, U, s.unit_blocks, s.stream, &err)) throw std::runtime_error(err);
+    hip_check(hipEventRecord(s.ev[1], s.stream), "event");
+  } else {
+    if (!launch_trace_samples(s.scene, L, e.d_samples, s.stream, &err)) throw std::runtime_error(err);
+    hip_check(hipEventRecord(s.ev[1], s.stream), "event");
+    if (!launch_reduce(L, e.d_samples, e.d_colours, s.stream, &err)) throw std::runtime_error(err);
+  }
   hip_check(hipEventRecord(s.ev[2], s.stream), "event");
   bool ok = true;
   if (e.shader == "single_dimension")
@@ -259,7 +334,7 @@ double render_compound(EyeState& e, const Cam& c) {
   s.timings[3] = (double)f[0] + f[1] + f[2];
   e.frame += 1;
   e.has_render = true;
-  e.rendered_rays = total;
+  e.rendered_rays = (record || !units) ? total : 0;
   s.frame_host_stale = true;
   return ms;
 }
@@ -562,6 +637,7 @@ void setCurrentEyeSamplesPerOmmatidium(int s) {
   if (!e) return;
   e->spp = std::max(1, s);
   e->restart_streams();
+  e->device_dirty = true;   // per-sample record offsets depend on the count
 }
 
 int getCurrentEyeSamplesPerOmmatidium(void) {
@@ -856,10 +932,10 @@ double renderPoseBatch(const float* poses, size_t poseCount, float* rgbOut) {
     pose_rotate_around(&p, q[5], 0.0f, 0.0f, 1.0f);
     pose_move(&p, q[0], q[1], q[2]);
   }
-  EyeLaunch L = launch_of(*e, nullptr, (int)poseCount);
+  UnitLaunch U = unit_launch_of(*e, nullptr, (int)poseCount);
   std::string err;
   double ms = -1.0;
-  if (!s.batch.render(s.scene, L, ps, rgbOut, s.stream, &ms, &err)) throw std::runtime_error(err);
+  if (!s.batch.render(s.scene, U, ps, rgbOut, s.stream, &ms, &err)) throw std::runtime_error(err);
   return ms;
   ABI_GUARD_END(return -1.0)
 }
@@ -989,6 +1065,39 @@ int setSceneLeafSize(int leafSize) {
   if (s.loaded) build_scene_bvh();
   return 0;
   ABI_GUARD_END(return -1)
+}
+
+void setTraversalMode(int mode) { set_trace_mode(mode == 0 ? kTracePerRay : kTracePacket); }
+
+void setSampleRecording(bool enabled) { st().record_samples = enabled; }
+
+void setRenderTuning(int unitsPerGrab, int blocks) {
+  State& s = st();
+  s.units_per_grab = std::max(1, std::min(unitsPerGrab, 1024));
+  s.unit_blocks = std::max(0, blocks);
+}
+
+double runCopyProbe(size_t bytes, int repeats) {
+  ABI_GUARD_BEGIN
+  ensure_hip();
+  const size_t n = bytes / sizeof(float4);
+  if (n == 0 || repeats < 1) return 0.0;
+  DeviceTemp tmp;
+  float4* src = tmp.upload<float4>(nullptr, n);
+  float4* dst = tmp.upload<float4>(nullptr, n);
+  State& s = st();
+  hip_check(hipMemsetAsync(src, 0, n * sizeof(float4), s.stream), "memset");
+  std::string err;
+  if (!launch_copy_probe(src, dst, n, s.stream, &err)) throw std::runtime_error(err);  // warm-up
+  hip_check(hipEventRecord(s.ev[0], s.stream), "event");
+  for (int r = 0; r < repeats; ++r)
+    if (!launch_copy_probe(src, dst, n, s.stream, &err)) throw std::runtime_error(err);
+  hip_check(hipEventRecord(s.ev[1], s.stream), "event");
+  hip_check(hipEventSynchronize(s.ev[1]), "sync");
+  float ms = 0.0f;
+  hip_check(hipEventElapsedTime(&ms, s.ev[0], s.ev[1]), "elapsed");
+  return ms / repeats;
+  ABI_GUARD_END(return -1.0)
 }
 
 int64_t parseGlTFTriangles(const char* path, float* pos, float* col, uint32_t* mode, size_t maxTris) {

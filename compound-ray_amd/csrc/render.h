@@ -26,13 +26,49 @@ struct EyeLaunch {
   uint32_t frame = 0;
 };
 
-struct TraceStats {
-  double ms = 0.0;
+// Persistent ommatidium-unit renderer.  A unit is one ommatidium at one pose;
+// waves take units from a work queue (`units_per_grab` at a time), trace a
+// unit's samples 64 at a time as one wave packet and reduce them in registers
+// in the canonical order (lane l sums samples l, l+64, ... then an xor
+// butterfly), so no per-sample buffer is needed.
+struct UnitLaunch {
+  const Ommatidium* omms = nullptr;
+  int omm_count = 0;
+  const Pose* poses = nullptr;
+  int pose_count = 1;
+  uint32_t pose_base = 0;
+  uint32_t seed = 42, eye = 0, frame = 0;
+  int spp = 1;                          // uniform count when both count arrays are null
+  const uint32_t* omm_counts = nullptr;   // per ommatidium, shared by every pose
+  const uint32_t* unit_counts = nullptr;  // per (pose, ommatidium), pose-major
+  uint32_t sample_base = 0;             // sample index of each unit's first sample
+  const uint32_t* order = nullptr;      // processing order of the ommatidia (null: index order)
+  float4* colours = nullptr;            // [pose][omm] = (r, g, b, n)
+  float4* stats = nullptr;              // nullable, [pose][omm][2]: (var r, var g, var b, hit fraction),
+                                        //   (mean inverse depth, its variance, 0, 0)
+  float4* samples = nullptr;            // nullable: per-sample (r, g, b, t or -1)
+  const uint64_t* sample_offsets = nullptr;  // first sample record of each unit
+  uint32_t* queue = nullptr;            // device work counter
+  int units_per_grab = 4;
 };
+
+// Launches the unit renderer on `blocks` blocks of 256 threads (0: one block
+// per (CU x occupancy) slot).  The queue counter is cleared by a kernel first.
+bool launch_render_units(const DeviceScene& scene, const UnitLaunch& U, int blocks, hipStream_t stream,
+                         std::string* err);
+
+// Traversal of eye rays: one ray per lane with its own LDS stack, or the
+// wave-packet traversal (coherent rays of one ommatidium share the walk).
+enum TraceMode : int { kTracePerRay = 0, kTracePacket = 1 };
+void set_trace_mode(int mode);
+int trace_mode();
 
 // Per-sample radiance: samples[ray] = (r, g, b, t) with t < 0 for a miss.
 bool launch_trace_samples(const DeviceScene& scene, const EyeLaunch& L, float4* samples,
                           hipStream_t stream, std::string* err);
+
+// Plain float4 copy of n elements (bandwidth reference for the profiler).
+bool launch_copy_probe(const float4* src, float4* dst, size_t n, hipStream_t stream, std::string* err);
 
 // Traversal work of the rays of L: totals[0] += nodes visited, totals[1] +=
 // triangles tested, totals[2] += rays that hit (device counters, zeroed by the caller).

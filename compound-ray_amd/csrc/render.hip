@@ -4,6 +4,7 @@
 // and the per-ommatidium wave reduction + projection (shaders.cu:341-406).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <string>
 
 #include "render.h"
@@ -193,6 +194,222 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace_samples(SceneView S, Ey
   samples[r] = make_float4(c.x, c.y, c.z, h.id == 0xffffffffu ? -1.0f : h.t);
 }
 
+// ---------------------------------------------------------------------------
+// Wave-packet traversal.  The 64 rays of a wave are samples of one or two
+// neighbouring ommatidia: nearly the same origin and direction.  The wave
+// walks the BVH together: the node index is wave-uniform (scalar loads, one
+// 64-entry stack per wave in LDS), each lane tests the children against its
+// own ray and closest hit, a child is entered when any lane's ray enters it,
+// and every active lane tests the triangles of an entered leaf.  Each lane
+// still computes the exact lexicographic (t, id) minimum over the triangles
+// its ray can hit (a lane only ever visits a superset of the nodes its own
+// traversal would), so results are identical to per-ray traversal.
+// ---------------------------------------------------------------------------
+template <bool kParallel>
+__device__ __forceinline__ void closest_hit_packet_impl(const SceneView& S, const RayPre& rp, bool active,
+                                                        int* wstack, Hit& h) {
+  int sp = 0;
+  int node = 0;
+  for (;;) {
+    node = __builtin_amdgcn_readfirstlane(node);
+    const float4* np = reinterpret_cast<const float4*>(S.nodes + node);
+    const float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5];
+    const int4 ch = reinterpret_cast<const int4*>(np)[6];
+    const float limit = h.t * kCullSlack;
+    float te[4];
+    te[0] = slab_enter<kParallel>(rp, lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, limit);
+    te[1] = slab_enter<kParallel>(rp, lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, limit);
+    te[2] = slab_enter<kParallel>(rp, lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, limit);
+    te[3] = slab_enter<kParallel>(rp, lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, limit);
+    const int32_t cc[4] = {ch.x, ch.y, ch.z, ch.w};
+    int ni = 0;
+    int ic[4];
+    float it[4];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      const int32_t c = cc[s];
+      const bool mine = active && te[s] < INFINITY;
+      if (c == kChildEmpty || __ballot(mine) == 0ull) continue;
+      if (child_is_leaf(c)) {
+        if (active) test_leaf(S.tris, c, rp, h);
+      } else {
+        // order key: the entry distance of the first lane that enters the child
+        const uint64_t m = __ballot(mine);
+        const int lead = __ffsll((long long)m) - 1;
+        ic[ni] = c;
+        it[ni] = __shfl(te[s], lead);
+        ++ni;
+      }
+    }
+    if (ni == 0) {
+      if (sp == 0) break;
+      --sp;
+      node = wstack[sp];
+      continue;
+    }
+    for (int a = 1; a < ni; ++a) {
+      const float tv = it[a];
+      const int cv = ic[a];
+      int b = a - 1;
+      while (b >= 0 && it[b] > tv) {
+        it[b + 1] = it[b];
+        ic[b + 1] = ic[b];
+        --b;
+      }
+      it[b + 1] = tv;
+      ic[b + 1] = cv;
+    }
+    // every lane holds the same children: lane 0 writes the wave's stack
+    if ((threadIdx.x & 63) == 0)
+      for (int a = ni - 1; a >= 1; --a) wstack[sp + (ni - 1 - a)] = ic[a];
+    sp += ni - 1;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    node = ic[0];
+  }
+}
+
+__device__ __forceinline__ void closest_hit_packet(const SceneView& S, const RayPre& rp, bool active, int* wstack,
+                                                   Hit& h) {
+  h.t = rp.tmax;
+  h.b1 = h.b2 = 0.0f;
+  h.k = 0xffffffffu;
+  h.id = 0xffffffffu;
+  if (S.node_count == 0) return;
+  if (__ballot(active && rp.par != 0) != 0ull)
+    closest_hit_packet_impl<true>(S, rp, active, wstack, h);
+  else
+    closest_hit_packet_impl<false>(S, rp, active, wstack, h);
+}
+
+__global__ __launch_bounds__(kTraceThreads) void k_trace_samples_packet(SceneView S, EyeLaunch L, uint64_t total,
+                                                                        float4* __restrict__ samples, int wstride) {
+  extern __shared__ int wave_stacks[];
+  const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool active = r < total;
+  int p = 0, i = 0, s = 0, n = 1;
+  if (active) locate_ray(L, r, &p, &i, &s, &n);
+  Ray ray;
+  if (active) {
+    const U4 rnd = sample_randoms(L.seed, L.eye, (uint32_t)i, (uint32_t)s, L.frame, L.pose_base + (uint32_t)p);
+    ray = generate_ray(L.omms[i], L.poses[p], rnd);
+  } else {
+    ray.ox = ray.oy = ray.oz = 0.0f;
+    ray.dx = ray.dy = 0.0f;
+    ray.dz = 1.0f;
+    ray.tmin = 0.0f;
+    ray.tmax = 1.0f;
+  }
+  const RayPre rp = precompute_ray(ray);
+  Hit h;
+  closest_hit_packet(S, rp, active, wave_stacks + (threadIdx.x >> 6) * wstride, h);
+  if (!active) return;
+  const V3 c = shade(S, h, ray.dx, ray.dy, ray.dz);
+  samples[r] = make_float4(c.x, c.y, c.z, h.id == 0xffffffffu ? -1.0f : h.t);
+}
+
+__device__ __forceinline__ float wave_sum(float v) {
+  for (int m = 32; m >= 1; m >>= 1) v = v + __shfl_xor(v, m);
+  return v;
+}
+
+__device__ __forceinline__ uint32_t unit_count(const UnitLaunch& U, uint64_t u, int i) {
+  if (U.unit_counts) return U.unit_counts[u];
+  if (U.omm_counts) return U.omm_counts[i];
+  return (uint32_t)U.spp;
+}
+
+template <bool kStats, bool kSamples>
+__global__ __launch_bounds__(kTraceThreads) void k_render_units(SceneView S, UnitLaunch U, int wstride) {
+  extern __shared__ int wave_stacks[];
+  const int lane = threadIdx.x & 63;
+  int* wst = wave_stacks + (threadIdx.x >> 6) * wstride;
+  const uint64_t units = (uint64_t)U.pose_count * (uint64_t)U.omm_count;
+  for (;;) {
+    uint32_t base = 0;
+    if (lane == 0) base = atomicAdd(U.queue, (uint32_t)U.units_per_grab);
+    base = __builtin_amdgcn_readfirstlane(__shfl(base, 0));
+    if ((uint64_t)base >= units) break;
+    for (int g = 0; g < U.units_per_grab; ++g) {
+      const uint64_t u = (uint64_t)base + (uint64_t)g;
+      if (u >= units) break;
+      const int p = (int)(u / (uint64_t)U.omm_count);
+      const int k = (int)(u - (uint64_t)p * (uint64_t)U.omm_count);
+      const int i = U.order ? (int)U.order[k] : k;
+      const uint64_t out = (uint64_t)p * (uint64_t)U.omm_count + (uint64_t)i;
+      const uint32_t n = unit_count(U, out, i);
+      const float inv = 1.0f / (float)(n > 0u ? n : 1u);
+      const Ommatidium o = U.omms[i];
+      const Pose pose = U.poses[p];
+      float r = 0.0f, gg = 0.0f, b = 0.0f;
+      float sr = 0.0f, sg = 0.0f, sb = 0.0f, sd = 0.0f, sd2 = 0.0f, sh = 0.0f;
+      for (uint32_t c0 = 0; c0 < n; c0 += 64u) {
+        const uint32_t s = c0 + (uint32_t)lane;
+        const bool active = s < n;
+        Ray ray;
+        if (active) {
+          const U4 rnd = sample_randoms(U.seed, U.eye, (uint32_t)i, U.sample_base + s, U.frame, U.pose_base + (uint32_t)p);
+          ray = generate_ray(o, pose, rnd);
+        } else {
+          ray.ox = ray.oy = ray.oz = 0.0f;
+          ray.dx = ray.dy = 0.0f;
+          ray.dz = 1.0f;
+          ray.tmin = 0.0f;
+          ray.tmax = 1.0f;
+        }
+        const RayPre rp = precompute_ray(ray);
+        Hit h;
+        closest_hit_packet(S, rp, active, wst, h);
+        if (active) {
+          const V3 c = shade(S, h, ray.dx, ray.dy, ray.dz);
+          r = r + c.x * inv;
+          gg = gg + c.y * inv;
+          b = b + c.z * inv;
+          const bool hit = h.id != 0xffffffffu;
+          if (kSamples) U.samples[U.sample_offsets[out] + s] = make_float4(c.x, c.y, c.z, hit ? h.t : -1.0f);
+          if (kStats) {
+            sr += c.x * c.x;
+            sg += c.y * c.y;
+            sb += c.z * c.z;
+            const float idp = hit ? 1.0f / (1.0f + h.t) : 0.0f;
+            sd += idp;
+            sd2 += idp * idp;
+            sh += hit ? 1.0f : 0.0f;
+          }
+        }
+      }
+      r = wave_sum(r);
+      gg = wave_sum(gg);
+      b = wave_sum(b);
+      if (kStats) {
+        sr = wave_sum(sr);
+        sg = wave_sum(sg);
+        sb = wave_sum(sb);
+        sd = wave_sum(sd);
+        sd2 = wave_sum(sd2);
+        sh = wave_sum(sh);
+      }
+      if (lane == 0) {
+        U.colours[out] = make_float4(r, gg, b, (float)n);
+        if (kStats) {
+          // E[c^2] - E[c]^2 with E[c] the unit's mean colour
+          const float vr = fmaxf(sr * inv - r * r, 0.0f);
+          const float vg = fmaxf(sg * inv - gg * gg, 0.0f);
+          const float vb = fmaxf(sb * inv - b * b, 0.0f);
+          const float md = sd * inv;
+          U.stats[2 * out] = make_float4(vr, vg, vb, sh * inv);
+          U.stats[2 * out + 1] = make_float4(md, fmaxf(sd2 * inv - md * md, 0.0f), 0.0f, 0.0f);
+        }
+      }
+    }
+  }
+}
+
+__global__ void k_fill_u32(uint32_t* __restrict__ p, uint32_t n, uint32_t v) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
 // Traversal work counters of the same rays (nodes visited, triangles tested),
 // summed over the launch: the algorithmic memory traffic of the trace kernel.
 __global__ __launch_bounds__(kTraceThreads) void k_trace_stats(SceneView S, EyeLaunch L, uint64_t total,
@@ -309,6 +526,11 @@ __global__ void k_frame_raw(const float4* __restrict__ samples, int omm_count, i
   frame[(size_t)y * width + x] = make_color(make_float4(v.x * inv, v.y * inv, v.z * inv, 0.0f));
 }
 
+__global__ void k_copy_probe(const float4* __restrict__ src, float4* __restrict__ dst, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
 bool check_launch(const char* what, std::string* err) {
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) {
@@ -325,10 +547,63 @@ size_t stack_bytes(const DeviceScene& s) {
 
 }  // namespace
 
+bool launch_render_units(const DeviceScene& scene, const UnitLaunch& U, int blocks, hipStream_t stream,
+                         std::string* err) {
+  const uint64_t units = (uint64_t)U.pose_count * (uint64_t)U.omm_count;
+  if (units == 0) return true;
+  if (units > 0xffffffffull - 65536ull) {
+    if (err) *err = "render_units: too many units for one launch";
+    return false;
+  }
+  if (!U.queue || !U.colours || U.units_per_grab < 1 || (U.samples && !U.sample_offsets)) {
+    if (err) *err = "render_units: incomplete launch description";
+    return false;
+  }
+  const int wstride = scene.stack_bound > 0 ? scene.stack_bound : 1;
+  const size_t lds = sizeof(int) * (size_t)wstride * (kTraceThreads / 64);
+  if (blocks <= 0) {
+    static int cus = 0;
+    if (cus == 0) {
+      int dev = 0;
+      (void)hipGetDevice(&dev);
+      if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+        cus = 256;
+    }
+    blocks = cus * 8;
+  }
+  const uint64_t need = (units + (uint64_t)U.units_per_grab - 1) / (uint64_t)U.units_per_grab;
+  const uint64_t waves = (uint64_t)blocks * (kTraceThreads / 64);
+  if (waves > need) blocks = (int)((need + (kTraceThreads / 64) - 1) / (kTraceThreads / 64));
+  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, stream, U.queue, 1u, 0u);
+  const SceneView v = view_of(scene);
+  if (U.stats && U.samples)
+    hipLaunchKernelGGL((k_render_units<true, true>), dim3(blocks), dim3(kTraceThreads), lds, stream, v, U, wstride);
+  else if (U.stats)
+    hipLaunchKernelGGL((k_render_units<true, false>), dim3(blocks), dim3(kTraceThreads), lds, stream, v, U, wstride);
+  else if (U.samples)
+    hipLaunchKernelGGL((k_render_units<false, true>), dim3(blocks), dim3(kTraceThreads), lds, stream, v, U, wstride);
+  else
+    hipLaunchKernelGGL((k_render_units<false, false>), dim3(blocks), dim3(kTraceThreads), lds, stream, v, U, wstride);
+  return check_launch("render_units", err);
+}
+
+static int g_trace_mode = kTracePacket;
+
+void set_trace_mode(int mode) { g_trace_mode = mode; }
+int trace_mode() { return g_trace_mode; }
+
 bool launch_trace_samples(const DeviceScene& scene, const EyeLaunch& L, float4* samples, hipStream_t stream,
                           std::string* err) {
   const uint64_t total = L.rays_per_pose * (uint64_t)L.pose_count;
   if (total == 0) return true;
+  if (g_trace_mode == kTracePacket) {
+    const int wstride = scene.stack_bound > 0 ? scene.stack_bound : 1;
+    const size_t lds = sizeof(int) * (size_t)wstride * (kTraceThreads / 64);
+    const uint64_t blocks = (total + kTraceThreads - 1) / kTraceThreads;
+    hipLaunchKernelGGL(k_trace_samples_packet, dim3((unsigned)blocks), dim3(kTraceThreads), lds, stream,
+                       view_of(scene), L, total, samples, wstride);
+    return check_launch("trace_samples_packet", err);
+  }
   const size_t lds = stack_bytes(scene);
   if (lds > 160 * 1024) {
     if (err) *err = "trace: BVH too deep for the LDS traversal stack";
@@ -338,6 +613,13 @@ bool launch_trace_samples(const DeviceScene& scene, const EyeLaunch& L, float4* 
   hipLaunchKernelGGL(k_trace_samples, dim3((unsigned)blocks), dim3(kTraceThreads), lds, stream, view_of(scene), L,
                      total, samples);
   return check_launch("trace_samples", err);
+}
+
+bool launch_copy_probe(const float4* src, float4* dst, size_t n, hipStream_t stream, std::string* err) {
+  if (n == 0) return true;
+  const size_t blocks = std::min<size_t>((n + 255) / 256, 256 * 32);
+  hipLaunchKernelGGL(k_copy_probe, dim3((unsigned)blocks), dim3(256), 0, stream, src, dst, n);
+  return check_launch("copy_probe", err);
 }
 
 bool launch_trace_stats(const DeviceScene& scene, const EyeLaunch& L, unsigned long long* totals,

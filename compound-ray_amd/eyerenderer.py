@@ -126,6 +126,18 @@ def configure_extensions(lib):
     lib.traceRays.restype = c_int
     lib.renderPoseBatch.argtypes = [f32p, c_size_t, c_void_p]
     lib.renderPoseBatch.restype = c_double
+    lib.setTraversalMode.argtypes = [c_int]
+    lib.setSampleRecording.argtypes = [c_bool]
+    lib.setRenderTuning.argtypes = [c_int, c_int]
+    lib.runCopyProbe.argtypes = [c_size_t, c_int]
+    lib.runCopyProbe.restype = c_double
+    lib.mlpParamCount.restype = c_size_t
+    lib.mlpInit.argtypes = [c_uint32]
+    lib.mlpSetParams.argtypes = [c_void_p, c_size_t]
+    lib.mlpGetParams.argtypes = [c_void_p, c_size_t]
+    lib.mlpPredict.argtypes = [c_void_p, c_size_t, c_void_p]
+    lib.mlpLossGrad.argtypes = [c_void_p, c_void_p, c_size_t, c_void_p, c_void_p]
+    lib.mlpTrain.argtypes = [c_void_p, c_void_p, c_size_t, c_int, c_int, c_float, c_uint32, c_void_p]
     lib.getSceneBVHInfo.argtypes = [i64p]
     lib.getSceneBVHInfo.restype = c_int
     lib.copySceneBVHNodes.argtypes = [c_void_p, c_size_t]

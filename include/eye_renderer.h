@@ -184,6 +184,21 @@ size_t copySceneSortOrder(uint32_t* ids, uint32_t* codes, size_t count);
 /* Rebuilds the BVH of the loaded scene with another leaf size (1..16). */
 int setSceneLeafSize(int leafSize);
 
+/* Traversal strategy of eye rays: 0 = one ray per lane, 1 = wave packets
+ * (default).  Results are identical; only speed differs. */
+void setTraversalMode(int mode);
+/* Keep the per-sample records of renderFrame for getCurrentEyeSamples (off by
+ * default: the packet renderer reduces samples in registers). */
+void setSampleRecording(bool enabled);
+/* Launch shape of the persistent unit renderer: ommatidia taken from the work
+ * queue per grab, and blocks of 256 threads (0 = 8 per compute unit). */
+void setRenderTuning(int unitsPerGrab, int blocks);
+
+/* Bandwidth reference for profiling: copies `bytes` (rounded down to 16-byte
+ * units) between two device buffers with a plain vector copy kernel
+ * (k_copy_probe), `repeats` times; returns the mean GPU ms per copy or -1. */
+double runCopyProbe(size_t bytes, int repeats);
+
 /* Host-only glTF parse (no GPU): world-space triangles (9 floats), colours
  * (9 floats) and colour modes of `path`.  Returns the triangle count (call
  * with maxTris = 0 to query), or -1 on error. */

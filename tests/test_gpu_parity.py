@@ -22,7 +22,8 @@ def load(lib, path):
     er.gotoFirstCompoundEye(lib)
 
 
-def setup_eye(lib, cfg, omm=None, spp=32, pose6=None):
+def setup_eye(lib, cfg, omm=None, spp=32, pose6=None, record=True):
+    lib.setSampleRecording(record)
     load(lib, cfg["gltf"])
     if omm is not None:
         er.set_ommatidia_np(lib, omm)
@@ -107,6 +108,54 @@ def test_render_bit_exact_c1(lib, config1, oracle_c1):
     assert np.array_equal(col, ref["colour_gpu_order"][0])
     # the reference's own (sequential) summation order: within 1e-5 relative
     assert rel_close(col, ref["colour_seq"][0], 1e-5)
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+@pytest.mark.parametrize("spp", [7, 32, 64, 100])
+def test_traversal_modes_bit_exact(lib, config1, oracle_c1, mode, spp):
+    """Per-ray and wave-packet traversal give the oracle's samples exactly."""
+    lib.setTraversalMode(mode)
+    try:
+        omm = config1["eyes"][0][::3]
+        setup_eye(lib, config1, omm=omm, spp=spp)
+        assert lib.renderFrame() >= 0, er.last_error(lib)
+        ref = oracle_c1.render(omm, er.pose_np(lib), spp=spp, want_samples=True)
+        assert np.array_equal(er.samples_np(lib).view(np.uint32), ref["samples"].view(np.uint32))
+        assert np.array_equal(er.colours_np(lib), ref["colour_gpu_order"][0])
+    finally:
+        lib.setTraversalMode(1)
+
+
+def test_unrecorded_render_matches(lib, config1, oracle_c1):
+    """Default renderFrame path (samples reduced in registers, nothing recorded)."""
+    setup_eye(lib, config1, spp=48, record=False)
+    assert lib.renderFrame() >= 0, er.last_error(lib)
+    ref = oracle_c1.render(config1["eyes"][0], er.pose_np(lib), spp=48)
+    assert np.array_equal(er.colours_np(lib), ref["colour_gpu_order"][0])
+    assert er.samples_np(lib).shape[0] == 0
+    lib.setSampleRecording(True)
+
+
+@pytest.mark.parametrize("grab,blocks", [(1, 1), (3, 7), (16, 0)])
+def test_launch_shapes_same_result(lib, config1, oracle_c1, grab, blocks):
+    omm = config1["eyes"][0][:400]
+    setup_eye(lib, config1, omm=omm, spp=24)
+    lib.setRenderTuning(grab, blocks)
+    try:
+        assert lib.renderFrame() >= 0, er.last_error(lib)
+        ref = oracle_c1.render(omm, er.pose_np(lib), spp=24)
+        assert np.array_equal(er.colours_np(lib), ref["colour_gpu_order"][0])
+    finally:
+        lib.setRenderTuning(4, 0)
+
+
+def test_traversal_stats_match_oracle(lib, config1, oracle_c1):
+    omm = config1["eyes"][0][:300]
+    setup_eye(lib, config1, omm=omm, spp=16)
+    st = er.traversal_stats(lib)
+    ref = oracle_c1.render(omm, er.pose_np(lib), spp=16)
+    assert st["nodes_visited"] == ref["nodes_visited"]
+    assert st["tris_tested"] == ref["tris_tested"]
 
 
 def test_frame_buffer_make_color(lib, config1, oracle_c1):
