@@ -28,6 +28,7 @@ SOURCES = [
     "render.hip",
     "batch.cpp",
     "mlp.hip",
+    "adaptive.hip",
     "eye_renderer.cpp",
 ]
 

@@ -1,0 +1,314 @@
+// Learned adaptive sampling on gfx950: pilot statistics -> features -> MLP ->
+// per-pose Neyman allocation -> top-up render -> combination.  Every step is a
+// kernel on one stream (no host round trip inside a batch).
+#include "adaptive.h"
+
+#include <algorithm>
+#include <cmath>
+
+namespace eye {
+namespace {
+
+constexpr int kThreads = 256;
+
+__global__ void k_features(const Ommatidium* __restrict__ omms, const Pose* __restrict__ poses, int omm_count,
+                           uint64_t units, const float4* __restrict__ col, const float4* __restrict__ stats,
+                           float* __restrict__ feat) {
+  const uint64_t u = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (u >= units) return;
+  const int p = (int)(u / (uint64_t)omm_count);
+  const int i = (int)(u - (uint64_t)p * (uint64_t)omm_count);
+  const Ommatidium o = omms[i];
+  const Pose P = poses[p];
+  const V3 a = normalize(v3(o.dx, o.dy, o.dz));
+  const V3 w = normalize((v3(P.x[0], P.x[1], P.x[2]) * a.x + v3(P.y[0], P.y[1], P.y[2]) * a.y) +
+                         v3(P.z[0], P.z[1], P.z[2]) * a.z);
+  const float4 c = col[u];
+  const float4 s0 = stats[2 * u];
+  const float4 s1 = stats[2 * u + 1];
+  float* f = feat + u * kMlpIn;
+  f[0] = w.x;
+  f[1] = w.y;
+  f[2] = w.z;
+  f[3] = o.acceptance * 20.0f;
+  f[4] = c.x;
+  f[5] = c.y;
+  f[6] = c.z;
+  f[7] = sqrtf(s0.x);
+  f[8] = sqrtf(s0.y);
+  f[9] = sqrtf(s0.z);
+  f[10] = s1.x;
+  f[11] = sqrtf(s1.y);
+}
+
+__device__ inline float required_samples(float log2_nreq) {
+  return exp2f(fminf(fmaxf(log2_nreq, 0.0f), 20.0f));
+}
+
+// One block per pose: S = sum sqrt(n_req) over the pose's ommatidia (fixed
+// order: per-thread strided sums, then a tree in LDS), then the counts.
+__global__ __launch_bounds__(kThreads) void k_allocate(const float* __restrict__ pred, int omm_count, int pilot,
+                                                      int max_spp, float scale, uint32_t* __restrict__ counts,
+                                                      uint32_t* __restrict__ extra,
+                                                      unsigned long long* __restrict__ total) {
+  __shared__ float part[kThreads];
+  const uint64_t base = (uint64_t)blockIdx.x * (uint64_t)omm_count;
+  float s = 0.0f;
+  for (int i = threadIdx.x; i < omm_count; i += kThreads) s += sqrtf(required_samples(pred[base + i]));
+  part[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = kThreads / 2; w >= 1; w >>= 1) {
+    if (threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+    __syncthreads();
+  }
+  const float S = part[0];
+  unsigned long long mine = 0;
+  for (int i = threadIdx.x; i < omm_count; i += kThreads) {
+    const float want = sqrtf(required_samples(pred[base + i])) * S / (float)omm_count * scale;
+    const float c = fminf(fmaxf(ceilf(want), (float)pilot), (float)max_spp);
+    const uint32_t n = (uint32_t)c;
+    counts[base + i] = n;
+    extra[base + i] = n - (uint32_t)pilot;
+    mine += n;
+  }
+  atomicAdd(total, mine);
+}
+
+__global__ void k_combine(const float4* __restrict__ a, float4* __restrict__ b, const uint32_t* __restrict__ extra,
+                          int pilot, uint64_t units) {
+  const uint64_t u = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (u >= units) return;
+  const float4 ca = a[u];
+  const uint32_t e = extra[u];
+  if (e == 0) {
+    b[u] = make_float4(ca.x, ca.y, ca.z, (float)pilot);
+    return;
+  }
+  const float4 cb = b[u];
+  const float wa = (float)pilot, wb = (float)e, inv = 1.0f / (wa + wb);
+  b[u] = make_float4((wa * ca.x + wb * cb.x) * inv, (wa * ca.y + wb * cb.y) * inv, (wa * ca.z + wb * cb.z) * inv,
+                     wa + wb);
+}
+
+// Training labels from a long render's statistics: log2(max(n_req, 1)), n_req
+// = mean RGB variance / error^2, clamped to [0, 12].
+__global__ void k_labels(const float4* __restrict__ stats, uint64_t units, float inv_err2, float* __restrict__ lab) {
+  const uint64_t u = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (u >= units) return;
+  const float4 s = stats[2 * u];
+  const float var = (s.x + s.y + s.z) * (1.0f / 3.0f);
+  const float n = fmaxf(var * inv_err2, 1.0f);
+  lab[u] = fminf(log2f(n), 12.0f);
+}
+
+__global__ void k_zero_u64(unsigned long long* p) { *p = 0ull; }
+
+template <typename T>
+bool grow(T** p, size_t n, std::string* err) {
+  if (*p) (void)hipFree(*p);
+  *p = nullptr;
+  if (n == 0) return true;
+  const hipError_t e = hipMalloc(p, sizeof(T) * n);
+  if (e != hipSuccess) {
+    if (err) *err = std::string("adaptive: hipMalloc: ") + hipGetErrorString(e);
+    return false;
+  }
+  return true;
+}
+
+bool ok_launch(const char* what, std::string* err) {
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    if (err) *err = std::string("adaptive: ") + what + ": " + hipGetErrorString(e);
+    return false;
+  }
+  return true;
+}
+
+}  // namespace
+
+void AdaptiveRenderer::release() {
+  for (void* p : {(void*)d_poses_, (void*)d_col_a_, (void*)d_col_b_, (void*)d_stats_, (void*)d_feat_,
+                  (void*)d_pred_, (void*)d_counts_, (void*)d_extra_, (void*)d_total_, (void*)d_queue_})
+    if (p) (void)hipFree(p);
+  d_poses_ = nullptr;
+  d_col_a_ = d_col_b_ = d_stats_ = nullptr;
+  d_feat_ = d_pred_ = nullptr;
+  d_counts_ = d_extra_ = d_queue_ = nullptr;
+  d_total_ = nullptr;
+  poses_cap_ = units_cap_ = 0;
+  if (ev0_) (void)hipEventDestroy(ev0_);
+  if (ev1_) (void)hipEventDestroy(ev1_);
+  ev0_ = ev1_ = nullptr;
+}
+
+bool AdaptiveRenderer::ensure(size_t poses, size_t units, std::string* err) {
+  if (!ev0_ && (hipEventCreate(&ev0_) != hipSuccess || hipEventCreate(&ev1_) != hipSuccess)) {
+    if (err) *err = "adaptive: event creation failed";
+    return false;
+  }
+  if (!d_total_ && (!grow(&d_total_, 1, err) || !grow(&d_queue_, 4, err))) return false;
+  if (poses > poses_cap_) {
+    if (!grow(&d_poses_, poses, err)) return false;
+    poses_cap_ = poses;
+  }
+  if (units > units_cap_) {
+    if (!grow(&d_col_a_, units, err) || !grow(&d_col_b_, units, err) || !grow(&d_stats_, 2 * units, err) ||
+        !grow(&d_feat_, units * kMlpIn, err) || !grow(&d_pred_, units, err) || !grow(&d_counts_, units, err) ||
+        !grow(&d_extra_, units, err))
+      return false;
+    units_cap_ = units;
+  }
+  return true;
+}
+
+bool AdaptiveRenderer::render(const DeviceScene& scene, UnitLaunch U, const std::vector<Pose>& poses, Mlp& mlp,
+                              const AdaptiveConfig& cfg, float* rgb, uint32_t* counts, AdaptiveResult* res,
+                              hipStream_t stream, std::string* err) {
+  const size_t np = poses.size();
+  const size_t units = np * (size_t)U.omm_count;
+  if (units == 0) return true;
+  if (!mlp.ready()) {
+    if (err) *err = "adaptive: the sampler network is not initialised (mlpInit / mlpTrain)";
+    return false;
+  }
+  if (cfg.pilot < 1 || cfg.max_spp < cfg.pilot) {
+    if (err) *err = "adaptive: need 1 <= pilot <= max samples";
+    return false;
+  }
+  if (!ensure(np, units, err)) return false;
+  if (hipMemcpyAsync(d_poses_, poses.data(), sizeof(Pose) * np, hipMemcpyHostToDevice, stream) != hipSuccess) {
+    if (err) *err = "adaptive: pose upload failed";
+    return false;
+  }
+  (void)hipEventRecord(ev0_, stream);
+  UnitLaunch A = U;
+  A.poses = d_poses_;
+  A.pose_count = (int)np;
+  A.pose_base = 0;
+  A.spp = cfg.pilot;
+  A.omm_counts = nullptr;
+  A.unit_counts = nullptr;
+  A.sample_base = 0;
+  A.colours = d_col_a_;
+  A.stats = d_stats_;
+  A.samples = nullptr;
+  A.queue = d_queue_;
+  if (!launch_render_units(scene, A, 0, stream, err)) return false;
+  const unsigned gb = (unsigned)((units + kThreads - 1) / kThreads);
+  hipLaunchKernelGGL(k_features, dim3(gb), dim3(kThreads), 0, stream, U.omms, d_poses_, U.omm_count, (uint64_t)units,
+                     d_col_a_, d_stats_, d_feat_);
+  if (!ok_launch("features", err)) return false;
+  if (!mlp.predict(d_feat_, (int64_t)units, d_pred_, stream, err)) return false;
+  hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, stream, d_total_);
+  hipLaunchKernelGGL(k_allocate, dim3((unsigned)np), dim3(kThreads), 0, stream, d_pred_, U.omm_count, cfg.pilot,
+                     cfg.max_spp, cfg.budget_scale, d_counts_, d_extra_, d_total_);
+  if (!ok_launch("allocate", err)) return false;
+  UnitLaunch B = A;
+  B.unit_counts = d_extra_;
+  B.sample_base = (uint32_t)cfg.pilot;
+  B.colours = d_col_b_;
+  B.stats = nullptr;
+  if (!launch_render_units(scene, B, 0, stream, err)) return false;
+  hipLaunchKernelGGL(k_combine, dim3(gb), dim3(kThreads), 0, stream, d_col_a_, d_col_b_, d_extra_, cfg.pilot,
+                     (uint64_t)units);
+  if (!ok_launch("combine", err)) return false;
+  (void)hipEventRecord(ev1_, stream);
+  unsigned long long total = 0;
+  if (hipMemcpyAsync(&total, d_total_, sizeof(total), hipMemcpyDeviceToHost, stream) != hipSuccess ||
+      hipStreamSynchronize(stream) != hipSuccess) {
+    if (err) *err = "adaptive: render failed";
+    return false;
+  }
+  float ms = 0.0f;
+  (void)hipEventElapsedTime(&ms, ev0_, ev1_);
+  if (res) {
+    res->gpu_ms = ms;
+    res->total_samples = total;
+  }
+  if (rgb) {
+    std::vector<float4> tmp(units);
+    if (hipMemcpy(tmp.data(), d_col_b_, sizeof(float4) * units, hipMemcpyDeviceToHost) != hipSuccess) {
+      if (err) *err = "adaptive: readback failed";
+      return false;
+    }
+    for (size_t u = 0; u < units; ++u) {
+      rgb[3 * u] = tmp[u].x;
+      rgb[3 * u + 1] = tmp[u].y;
+      rgb[3 * u + 2] = tmp[u].z;
+    }
+  }
+  if (counts && hipMemcpy(counts, d_counts_, sizeof(uint32_t) * units, hipMemcpyDeviceToHost) != hipSuccess) {
+    if (err) *err = "adaptive: readback failed";
+    return false;
+  }
+  return true;
+}
+
+bool AdaptiveRenderer::training_set(const DeviceScene& scene, UnitLaunch U, const std::vector<Pose>& poses,
+                                    const std::vector<uint8_t>& unit_mask, int pilot, int label_spp,
+                                    float error_target, std::vector<float>* features, std::vector<float>* labels,
+                                    hipStream_t stream, std::string* err) {
+  const size_t np = poses.size();
+  const size_t units = np * (size_t)U.omm_count;
+  if (units == 0) return true;
+  if (unit_mask.size() != units || pilot < 1 || label_spp < 2 || !(error_target > 0.0f)) {
+    if (err) *err = "adaptive: bad training-set request";
+    return false;
+  }
+  if (!ensure(np, units, err)) return false;
+  if (hipMemcpyAsync(d_poses_, poses.data(), sizeof(Pose) * np, hipMemcpyHostToDevice, stream) != hipSuccess) {
+    if (err) *err = "adaptive: pose upload failed";
+    return false;
+  }
+  std::vector<uint32_t> pc(units), lc(units);
+  for (size_t u = 0; u < units; ++u) {
+    pc[u] = unit_mask[u] ? (uint32_t)pilot : 0u;
+    lc[u] = unit_mask[u] ? (uint32_t)label_spp : 0u;
+  }
+  if (hipMemcpyAsync(d_counts_, pc.data(), sizeof(uint32_t) * units, hipMemcpyHostToDevice, stream) != hipSuccess ||
+      hipMemcpyAsync(d_extra_, lc.data(), sizeof(uint32_t) * units, hipMemcpyHostToDevice, stream) != hipSuccess) {
+    if (err) *err = "adaptive: count upload failed";
+    return false;
+  }
+  UnitLaunch A = U;
+  A.poses = d_poses_;
+  A.pose_count = (int)np;
+  A.pose_base = 0;
+  A.omm_counts = nullptr;
+  A.unit_counts = d_counts_;
+  A.sample_base = 0;
+  A.colours = d_col_a_;
+  A.stats = d_stats_;
+  A.samples = nullptr;
+  A.queue = d_queue_;
+  if (!launch_render_units(scene, A, 0, stream, err)) return false;
+  const unsigned gb = (unsigned)((units + kThreads - 1) / kThreads);
+  hipLaunchKernelGGL(k_features, dim3(gb), dim3(kThreads), 0, stream, U.omms, d_poses_, U.omm_count, (uint64_t)units,
+                     d_col_a_, d_stats_, d_feat_);
+  if (!ok_launch("features", err)) return false;
+  // label render: independent samples pilot .. pilot + label_spp - 1
+  UnitLaunch B = A;
+  B.unit_counts = d_extra_;
+  B.sample_base = (uint32_t)pilot;
+  B.colours = d_col_b_;
+  if (!launch_render_units(scene, B, 0, stream, err)) return false;
+  hipLaunchKernelGGL(k_labels, dim3(gb), dim3(kThreads), 0, stream, d_stats_, (uint64_t)units,
+                     1.0f / (error_target * error_target), d_pred_);
+  if (!ok_launch("labels", err)) return false;
+  std::vector<float> f(units * kMlpIn), l(units);
+  if (hipMemcpyAsync(f.data(), d_feat_, sizeof(float) * f.size(), hipMemcpyDeviceToHost, stream) != hipSuccess ||
+      hipMemcpyAsync(l.data(), d_pred_, sizeof(float) * l.size(), hipMemcpyDeviceToHost, stream) != hipSuccess ||
+      hipStreamSynchronize(stream) != hipSuccess) {
+    if (err) *err = "adaptive: training-set readback failed";
+    return false;
+  }
+  for (size_t u = 0; u < units; ++u) {
+    if (!unit_mask[u]) continue;
+    features->insert(features->end(), f.begin() + (ptrdiff_t)(u * kMlpIn), f.begin() + (ptrdiff_t)((u + 1) * kMlpIn));
+    labels->push_back(l[u]);
+  }
+  return true;
+}
+
+}  // namespace eye

@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "../../include/eye_renderer.h"
+#include "adaptive.h"
 #include "batch.h"
 #include "device_scene.h"
 #include "mlp.h"
@@ -96,6 +97,7 @@ struct State {
   unsigned long long* d_stats = nullptr;
   BatchRenderer batch;
   Mlp mlp;
+  AdaptiveRenderer adaptive;
   bool record_samples = false;   // keep per-sample records of renderFrame
   uint32_t* d_queue = nullptr;   // work counter of the unit renderer
   int units_per_grab = 4;
@@ -1020,6 +1022,67 @@ int mlpTrain(const float* features, const float* targets, size_t rows, int epoch
   if (epochLosses) std::copy(losses.begin(), losses.end(), epochLosses);
   return 0;
   ABI_GUARD_END(return -1)
+}
+
+static std::vector<Pose> poses_from6(const float* poses, size_t n) {
+  std::vector<Pose> ps(n);
+  for (size_t k = 0; k < n; ++k) {
+    Pose& p = ps[k];
+    const float* q = poses + 6 * k;
+    pose_reset(&p);
+    pose_rotate_around(&p, q[3], 1.0f, 0.0f, 0.0f);
+    pose_rotate_around(&p, q[4], 0.0f, 1.0f, 0.0f);
+    pose_rotate_around(&p, q[5], 0.0f, 0.0f, 1.0f);
+    pose_move(&p, q[0], q[1], q[2]);
+  }
+  return ps;
+}
+
+int64_t adaptiveTrainingSet(const float* poses, size_t poseCount, const uint8_t* unitMask, int pilot, int labelSpp,
+                            float errorTarget, float* features, float* labels, size_t maxRows) {
+  ABI_GUARD_BEGIN
+  State& s = st();
+  EyeState* e = current_eye();
+  if (!s.loaded || !e) throw std::runtime_error("adaptiveTrainingSet: no compound eye selected");
+  if (!poses && poseCount) throw std::runtime_error("adaptiveTrainingSet: null poses");
+  upload_eye(*e);
+  const size_t units = poseCount * e->omms.size();
+  std::vector<uint8_t> mask(units, 1);
+  if (unitMask) mask.assign(unitMask, unitMask + units);
+  std::vector<float> f, l;
+  std::string err;
+  UnitLaunch U = unit_launch_of(*e, nullptr, (int)poseCount);
+  if (!s.adaptive.training_set(s.scene, U, poses_from6(poses, poseCount), mask, pilot, labelSpp, errorTarget, &f,
+                               &l, s.stream, &err))
+    throw std::runtime_error(err);
+  const size_t rows = std::min(l.size(), maxRows);
+  if (features) std::copy(f.begin(), f.begin() + (ptrdiff_t)(rows * kMlpIn), features);
+  if (labels) std::copy(l.begin(), l.begin() + (ptrdiff_t)rows, labels);
+  return (int64_t)rows;
+  ABI_GUARD_END(return -1)
+}
+
+double renderPoseBatchAdaptive(const float* poses, size_t poseCount, float errorTarget, int maxSpp,
+                               float budgetScale, float* rgbOut, uint32_t* countsOut, uint64_t* totalSamples) {
+  ABI_GUARD_BEGIN
+  State& s = st();
+  EyeState* e = current_eye();
+  if (!s.loaded || !e) throw std::runtime_error("renderPoseBatchAdaptive: no compound eye selected");
+  if (!poses && poseCount) throw std::runtime_error("renderPoseBatchAdaptive: null poses");
+  upload_eye(*e);
+  AdaptiveConfig cfg;
+  cfg.error_target = errorTarget;
+  cfg.max_spp = maxSpp;
+  cfg.budget_scale = budgetScale;
+  AdaptiveResult res;
+  std::string err;
+  UnitLaunch U = unit_launch_of(*e, nullptr, (int)poseCount);
+  if (!s.adaptive.render(s.scene, U, poses_from6(poses, poseCount), s.mlp, cfg, rgbOut, countsOut, &res, s.stream,
+                         &err))
+    throw std::runtime_error(err);
+  if (totalSamples) *totalSamples = res.total_samples;
+  return res.gpu_ms;
+  ABI_GUARD_END(return -1.0)
 }
 
 int getSceneBVHInfo(int64_t* info) {

@@ -126,6 +126,12 @@ def configure_extensions(lib):
     lib.traceRays.restype = c_int
     lib.renderPoseBatch.argtypes = [f32p, c_size_t, c_void_p]
     lib.renderPoseBatch.restype = c_double
+    lib.adaptiveTrainingSet.argtypes = [c_void_p, c_size_t, c_void_p, c_int, c_int, c_float, c_void_p, c_void_p,
+                                        c_size_t]
+    lib.adaptiveTrainingSet.restype = c_int64
+    lib.renderPoseBatchAdaptive.argtypes = [c_void_p, c_size_t, c_float, c_int, c_float, c_void_p, c_void_p,
+                                            c_void_p]
+    lib.renderPoseBatchAdaptive.restype = c_double
     lib.setTraversalMode.argtypes = [c_int]
     lib.setSampleRecording.argtypes = [c_bool]
     lib.setRenderTuning.argtypes = [c_int, c_int]
@@ -297,6 +303,55 @@ def render_batch_np(lib, poses6: np.ndarray, want_rgb: bool = True):
     if ms < 0:
         raise RuntimeError(last_error(lib))
     return ms, out
+
+
+def training_set_np(lib, poses6, unit_mask=None, pilot=8, label_spp=1024, error_target=0.01):
+    """(features (n, 12), labels (n,)) for the masked (pose, ommatidium) units."""
+    p = np.ascontiguousarray(poses6, dtype=np.float32).reshape(-1, 6)
+    n_omm = lib.getCurrentEyeOmmatidialCount()
+    m = None if unit_mask is None else np.ascontiguousarray(unit_mask, dtype=np.uint8).reshape(-1)
+    rows = int(m.sum()) if m is not None else p.shape[0] * n_omm
+    f = np.zeros((rows, 12), np.float32)
+    lab = np.zeros(rows, np.float32)
+    got = lib.adaptiveTrainingSet(p.ctypes.data_as(c_void_p), p.shape[0],
+                                  None if m is None else m.ctypes.data_as(c_void_p), pilot, label_spp,
+                                  error_target, f.ctypes.data_as(c_void_p), lab.ctypes.data_as(c_void_p), rows)
+    if got < 0:
+        raise RuntimeError(last_error(lib))
+    return f[:got], lab[:got]
+
+
+def mlp_train_np(lib, features, labels, epochs=10, batch=4096, lr=1e-3, seed=1):
+    x = np.ascontiguousarray(features, dtype=np.float32)
+    t = np.ascontiguousarray(labels, dtype=np.float32)
+    losses = np.zeros(epochs, np.float32)
+    if lib.mlpTrain(x.ctypes.data_as(c_void_p), t.ctypes.data_as(c_void_p), x.shape[0], epochs, batch, lr, seed,
+                    losses.ctypes.data_as(c_void_p)) != 0:
+        raise RuntimeError(last_error(lib))
+    return losses
+
+
+def mlp_predict_np(lib, features):
+    x = np.ascontiguousarray(features, dtype=np.float32)
+    y = np.zeros(x.shape[0], np.float32)
+    if lib.mlpPredict(x.ctypes.data_as(c_void_p), x.shape[0], y.ctypes.data_as(c_void_p)) != 0:
+        raise RuntimeError(last_error(lib))
+    return y
+
+
+def render_adaptive_np(lib, poses6, error_target=0.01, max_spp=512, budget_scale=1.0, want_rgb=True):
+    p = np.ascontiguousarray(poses6, dtype=np.float32).reshape(-1, 6)
+    n = lib.getCurrentEyeOmmatidialCount()
+    rgb = np.zeros((p.shape[0], n, 3), np.float32) if want_rgb else None
+    counts = np.zeros((p.shape[0], n), np.uint32) if want_rgb else None
+    total = np.zeros(1, np.uint64)
+    ms = lib.renderPoseBatchAdaptive(p.ctypes.data_as(c_void_p), p.shape[0], error_target, max_spp, budget_scale,
+                                     rgb.ctypes.data_as(c_void_p) if want_rgb else None,
+                                     counts.ctypes.data_as(c_void_p) if want_rgb else None,
+                                     total.ctypes.data_as(c_void_p))
+    if ms < 0:
+        raise RuntimeError(last_error(lib))
+    return ms, rgb, counts, int(total[0])
 
 
 def bvh_info(lib) -> dict:

@@ -174,6 +174,21 @@ int mlpLossGrad(const float* features, const float* targets, size_t rows, float*
 int mlpTrain(const float* features, const float* targets, size_t rows, int epochs, int batch, float lr,
              uint32_t seed, float* epochLosses);
 
+/* ---- Learned adaptive sampling of the current eye ----
+ * Training rows for the sampler: for every (pose, ommatidium) with
+ * unitMask[pose * ommatidia + omm] != 0 (NULL mask: all), 12 features from
+ * `pilot` samples and the label log2(max(1, mean RGB variance / errorTarget^2))
+ * from `labelSpp` further samples.  Writes at most maxRows rows (features
+ * 12 floats each); returns the number of rows, or -1. */
+int64_t adaptiveTrainingSet(const float* poses, size_t poseCount, const uint8_t* unitMask, int pilot, int labelSpp,
+                            float errorTarget, float* features, float* labels, size_t maxRows);
+/* Batched adaptive render: 8-sample pilot, MLP prediction, per-pose Neyman
+ * allocation for RMS error errorTarget (times budgetScale), top-up, combine.
+ * rgbOut / countsOut (nullable) per (pose, ommatidium); *totalSamples gets the
+ * samples traced.  Returns GPU milliseconds, or -1. */
+double renderPoseBatchAdaptive(const float* poses, size_t poseCount, float errorTarget, int maxSpp,
+                               float budgetScale, float* rgbOut, uint32_t* countsOut, uint64_t* totalSamples);
+
 /* Acceleration-structure inspection: info[0..7] = triangle count, node count,
  * stack bound, levels, leaf size, build ms * 1000, 0, 0.  Returns 0 or -1. */
 int getSceneBVHInfo(int64_t* info);

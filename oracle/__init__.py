@@ -46,8 +46,9 @@ def lib():
         L.oracle_generate_rays.argtypes = [c_void_p, c_int64, c_void_p, c_int, c_void_p, c_int64, c_uint32,
                                            c_uint32, c_uint32, c_void_p]
         L.oracle_trace.argtypes = [c_void_p, c_void_p, c_int64, c_int, c_void_p, c_void_p, c_void_p, c_int]
-        L.oracle_render.argtypes = [c_void_p, c_void_p, c_int64, c_void_p, c_int, c_void_p, c_int64, c_uint32,
-                                    c_uint32, c_uint32, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int]
+        L.oracle_render.argtypes = [c_void_p, c_void_p, c_int64, c_void_p, c_int, c_int, c_void_p, c_int64,
+                                    c_uint32, c_uint32, c_uint32, c_uint32, c_void_p, c_void_p, c_void_p, c_void_p,
+                                    c_void_p, c_int]
         L.oracle_make_color.argtypes = [c_void_p, c_int64, c_void_p]
         L.oracle_pose.argtypes = [c_void_p, c_void_p, c_void_p]
         _lib = L
@@ -144,19 +145,28 @@ class OracleScene:
         return (ids, tbb, vis) if visits else (ids, tbb)
 
     def render(self, omm, poses12, spp: int = 32, counts=None, seed: int = 42, eye: int = 0, frame: int = 0,
-               want_samples: bool = False, want_f64: bool = False, threads: int | None = None) -> dict:
+               want_samples: bool = False, want_f64: bool = False, threads: int | None = None,
+               sample_base: int = 0) -> dict:
+        """counts: per ommatidium (shared by the poses) or, with shape
+        (poses, ommatidia), per unit."""
         o = np.ascontiguousarray(omm, dtype=np.float32).reshape(-1, 8)
         P = np.ascontiguousarray(poses12, dtype=np.float32).reshape(-1, 12)
         c = None if counts is None else np.ascontiguousarray(counts, dtype=np.uint32)
         n_omm, n_pose = o.shape[0], P.shape[0]
-        per_pose = int(c.sum()) if c is not None else n_omm * spp
-        samples = np.zeros((n_pose * per_pose, 4), dtype=np.float32) if want_samples else None
+        per_unit = c is not None and c.ndim == 2
+        if per_unit:
+            assert c.shape == (n_pose, n_omm)
+            total = int(c.sum())
+        else:
+            total = n_pose * (int(c.sum()) if c is not None else n_omm * spp)
+        samples = np.zeros((total, 4), dtype=np.float32) if want_samples else None
         gpu = np.zeros((n_pose, n_omm, 3), dtype=np.float32)
         seq = np.zeros((n_pose, n_omm, 3), dtype=np.float32)
         f64 = np.zeros((n_pose, n_omm, 3), dtype=np.float64) if want_f64 else None
         visits = np.zeros(2, dtype=np.uint64)
-        lib().oracle_render(self.h, _p(o), n_omm, _p(c), spp, _p(P), n_pose, seed, eye, frame, _p(samples),
-                            _p(gpu), _p(seq), _p(f64), _p(visits), threads or default_threads())
+        lib().oracle_render(self.h, _p(o), n_omm, _p(c), 1 if per_unit else 0, spp, _p(P), n_pose, seed, eye,
+                            frame, sample_base, _p(samples), _p(gpu), _p(seq), _p(f64), _p(visits),
+                            threads or default_threads())
         return {"samples": samples, "colour_gpu_order": gpu, "colour_seq": seq, "colour_f64": f64,
                 "nodes_visited": int(visits[0]), "tris_tested": int(visits[1])}
 

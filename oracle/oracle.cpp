@@ -768,31 +768,37 @@ void oracle_trace(void* h, const float* rays, int64_t n, int mode, int32_t* ids,
 //   col_seq  3 floats per (pose, omm): the reference's sequential order
 //   col_f64  3 doubles per (pose, omm): FP64 accumulation of unscaled samples / n
 //   visits   2 uint64 totals (nodes visited, triangles tested)
-void oracle_render(void* h, const float* omms, int64_t n_omm, const uint32_t* counts, int spp, const float* poses,
-                   int64_t n_pose, uint32_t seed, uint32_t eye, uint32_t frame, float* samples, float* col_gpu,
-                   float* col_seq, double* col_f64, uint64_t* visits, int threads) {
+//   counts_per_unit: counts holds one entry per (pose, omm) instead of per omm
+//   sample_base:     sample index of each unit's first sample (top-up passes)
+void oracle_render(void* h, const float* omms, int64_t n_omm, const uint32_t* counts, int counts_per_unit, int spp,
+                   const float* poses, int64_t n_pose, uint32_t seed, uint32_t eye, uint32_t frame,
+                   uint32_t sample_base, float* samples, float* col_gpu, float* col_seq, double* col_f64,
+                   uint64_t* visits, int threads) {
   const Scene& S = *static_cast<Scene*>(h);
-  std::vector<uint64_t> first((size_t)n_omm + 1, 0);
-  for (int64_t i = 0; i < n_omm; ++i) first[(size_t)i + 1] = first[(size_t)i] + samples_of(counts, spp, (size_t)i);
-  const uint64_t per_pose = first[(size_t)n_omm];
+  const size_t units = (size_t)(n_omm * n_pose);
+  auto count_of = [&](size_t u) {
+    return counts_per_unit ? samples_of(counts, spp, u) : samples_of(counts, spp, u % (size_t)n_omm);
+  };
+  std::vector<uint64_t> first(units + 1, 0);
+  for (size_t u = 0; u < units; ++u) first[u + 1] = first[u] + count_of(u);
   std::atomic<uint64_t> tot_nodes(0), tot_tris(0);
-  parallel_for((size_t)(n_omm * n_pose), threads, [&](size_t u) {
+  parallel_for(units, threads, [&](size_t u) {
     const size_t p = u / (size_t)n_omm, i = u % (size_t)n_omm;
     const PoseO P = pose_from(poses + 12 * p);
     Omm om;
     std::memcpy(&om, omms + 8 * i, sizeof(Omm));
-    const uint32_t n = samples_of(counts, spp, i);
+    const uint32_t n = count_of(u);
     const float inv = 1.0f / (float)(n > 0 ? n : 1u);
     float lane[64][3] = {};
     float seq[3] = {0.0f, 0.0f, 0.0f};
     double f64[3] = {0.0, 0.0, 0.0};
     uint64_t nv = 0, nt = 0;
     for (uint32_t s = 0; s < n; ++s) {
-      const RayO r = make_ray(om, P, seed, eye, (uint32_t)i, s, frame, (uint32_t)p);
+      const RayO r = make_ray(om, P, seed, eye, (uint32_t)i, sample_base + s, frame, (uint32_t)p);
       const HitO hh = trace_bvh(S, r, &nv, &nt);
       const F3 c = shade(S, hh, r);
       if (samples) {
-        float* o = samples + 4 * (p * per_pose + first[i] + s);
+        float* o = samples + 4 * (first[u] + s);
         o[0] = c.x; o[1] = c.y; o[2] = c.z;
         o[3] = hh.id == 0xffffffffu ? -1.0f : hh.t;
       }
