@@ -33,6 +33,22 @@ METRIC = "eye-frames/s at ≤1% RMS colour err; Grays/s; learned-adaptive vs uni
 HBM_PEAK_GBS = 8000.0
 
 
+TRAFFIC_FILE = ROOT / "profiles" / "k_render_units_c2_traffic.txt"
+
+
+def measured_traffic():
+    """HBM bytes per k_render_units launch on this workload, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (already scaled by the copy-kernel
+    factors recorded there); None when no summary is present."""
+    try:
+        for line in TRAFFIC_FILE.read_text().splitlines():
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":", 1)[1])
+    except (OSError, ValueError):
+        pass
+    return None
+
+
 def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -124,6 +140,8 @@ def main() -> int:
     ap.add_argument("--profile", action="store_true",
                     help="render loop only (no 4096-spp error check, no CPU baseline): for profiler runs")
     ap.add_argument("--skip-batched", action="store_true", help="skip the 4,096-pose batched configuration")
+    ap.add_argument("--trace-mode", type=int, default=1, choices=(0, 1, 2),
+                    help="traversal: 0 per-ray, 1 wave packets (default), 2 wave packets with scalar node loads")
     ap.add_argument("--copy-probe", type=float, default=0.0,
                     help="also run a plain copy kernel of this many MiB (profiler bandwidth reference)")
     args = ap.parse_args()
@@ -137,6 +155,7 @@ def main() -> int:
 
     lib = cra.load_library()
     lib.setVerbosity(False)
+    lib.setTraversalMode(args.trace_mode)
     tmp = tempfile.mkdtemp(prefix="eyebench-")
     t0 = time.time()
     cfg = scenes.build_config(2, tmp)
@@ -189,12 +208,12 @@ def main() -> int:
     value = eye_frames / elapsed
     grays = args.steps * rays_per_step / elapsed / 1e9
     trace_avg_ms = float(np.mean(trace_ms))
-    # algorithmic bytes of one trace launch (one eye-frame): BVH nodes visited
+    # algorithmic bytes of one render launch (one eye-frame): BVH nodes visited
     # x 128 B + triangles tested x 48 B + shading fetch (3 x 16 B per hit) +
-    # ommatidium record (32 B / spp per ray) + per-sample record written (16 B)
-    rays_eye = rays_per_step / len(eyes)
+    # per ommatidium its 32 B record read and its 16 B colour written
+    omm_total = sum(omm_counts)
     bytes_launch = (stats["nodes_visited"] * 128 + stats["tris_tested"] * 48 + stats["hits"] * 48
-                    + rays_per_step * (32.0 / spp + 16.0)) / len(eyes)
+                    + omm_total * (32 + 16)) / len(eyes)
     achieved = bytes_launch / (trace_avg_ms * 1e-3) / 1e9
     if args.copy_probe > 0:
         nbytes = int(args.copy_probe * 2 ** 20)
@@ -239,8 +258,9 @@ def main() -> int:
                               "tris": stats["tris_tested"] / rays_per_step,
                               "hit_fraction": stats["hits"] / rays_per_step},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_render_units", "bytes_per_launch": bytes_launch},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(),
+                     "kernel": "k_render_units", "bytes_per_launch": bytes_launch,
+                     "trace_mode": args.trace_mode},
     }
     if not (args.no_cpu_baseline or args.profile):
         lib.gotoCamera(eyes[0])

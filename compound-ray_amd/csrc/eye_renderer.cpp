@@ -292,7 +292,7 @@ double render_compound(EyeState& e, const Cam& c) {
   const uint64_t total = e.total_samples();
   // per-sample records are kept only when asked for (or for the raw projection)
   const bool record = s.record_samples || e.shader == "raw_ommatidial_samples";
-  const bool units = trace_mode() == kTracePacket;
+  const bool units = trace_mode() != kTracePerRay;
   if (record || !units) ensure_device(&e.d_samples, &e.d_samples_cap, (size_t)total);
   ensure_device(&e.d_colours, &e.d_colours_cap, e.omms.size());
   ensure_frame_buffer();
@@ -1130,7 +1130,9 @@ int setSceneLeafSize(int leafSize) {
   ABI_GUARD_END(return -1)
 }
 
-void setTraversalMode(int mode) { set_trace_mode(mode == 0 ? kTracePerRay : kTracePacket); }
+void setTraversalMode(int mode) {
+  set_trace_mode(mode == 0 ? kTracePerRay : mode == 2 ? kTracePacketScalar : kTracePacket);
+}
 
 void setSampleRecording(bool enabled) { st().record_samples = enabled; }
 

@@ -58,8 +58,10 @@ bool launch_render_units(const DeviceScene& scene, const UnitLaunch& U, int bloc
                          std::string* err);
 
 // Traversal of eye rays: one ray per lane with its own LDS stack, or the
-// wave-packet traversal (coherent rays of one ommatidium share the walk).
-enum TraceMode : int { kTracePerRay = 0, kTracePacket = 1 };
+// wave-packet traversal (coherent rays of one ommatidium share the walk),
+// whose node and triangle records are read with vector loads or, in
+// kTracePacketScalar, through the scalar unit (the address is wave-uniform).
+enum TraceMode : int { kTracePerRay = 0, kTracePacket = 1, kTracePacketScalar = 2 };
 void set_trace_mode(int mode);
 int trace_mode();
 

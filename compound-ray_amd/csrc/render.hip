@@ -38,15 +38,39 @@ struct Hit {
   uint32_t id;  // original triangle id, 0xffffffff for a miss
 };
 
-__device__ __forceinline__ void test_leaf(const float4* __restrict__ tris, int32_t code, const RayPre& rp,
-                                          Hit& h) {
+// Pointers into the constant address space: wave-uniform loads through them
+// become scalar (SMEM) loads, fetched once per wave into SGPRs.  Only used for
+// data no kernel writes while it is read (nodes, triangles).
+typedef float nf4 __attribute__((ext_vector_type(4)));
+typedef int ni4 __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(4))) const nf4 kf4;
+typedef __attribute__((address_space(4))) const ni4 ki4;
+
+struct UniformPtr {   // float4 array read through the constant address space
+  const void* p;
+};
+struct VaryingPtr {   // float4 array read with per-lane (vector) loads
+  const float4* p;
+};
+__device__ __forceinline__ float4 load4(UniformPtr b, size_t i) {
+  const nf4 v = ((const kf4*)b.p)[i];
+  return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ float4 load4(VaryingPtr b, size_t i) { return b.p[i]; }
+__device__ __forceinline__ int4 load_int4_uniform(const void* p, size_t i) {
+  const ni4 v = ((const ki4*)p)[i];
+  return make_int4(v.x, v.y, v.z, v.w);
+}
+
+template <typename P4>
+__device__ __forceinline__ void test_leaf(P4 tris, int32_t code, const RayPre& rp, Hit& h) {
   const uint32_t first = leaf_first(code);
   const uint32_t cnt = leaf_count(code);
   for (uint32_t q = 0; q < cnt; ++q) {
     const uint32_t k = first + q;
-    const float4 a = tris[3 * (size_t)k + 0];
-    const float4 b = tris[3 * (size_t)k + 1];
-    const float4 c = tris[3 * (size_t)k + 2];
+    const float4 a = load4(tris, 3 * (size_t)k + 0);
+    const float4 b = load4(tris, 3 * (size_t)k + 1);
+    const float4 c = load4(tris, 3 * (size_t)k + 2);
     float t, b1, b2;
     if (intersect_triangle(rp, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), v3(c.x, c.y, c.z), &t, &b1, &b2)) {
       const uint32_t id = __float_as_uint(a.w);
@@ -96,7 +120,7 @@ __device__ __forceinline__ void closest_hit_impl(const SceneView& S, const RayPr
       if (c == kChildEmpty || !(te[s] < INFINITY)) continue;
       if (child_is_leaf(c)) {
         if (kStats) *tris_tested += leaf_count(c);
-        test_leaf(S.tris, c, rp, h);
+        test_leaf(VaryingPtr{S.tris}, c, rp, h);
       } else {
         ic[ni] = c;
         it[ni] = te[s];
@@ -205,16 +229,25 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace_samples(SceneView S, Ey
 // its ray can hit (a lane only ever visits a superset of the nodes its own
 // traversal would), so results are identical to per-ray traversal.
 // ---------------------------------------------------------------------------
-template <bool kParallel>
+template <bool kParallel, bool kScalar>
 __device__ __forceinline__ void closest_hit_packet_impl(const SceneView& S, const RayPre& rp, bool active,
                                                         int* wstack, Hit& h) {
   int sp = 0;
   int node = 0;
   for (;;) {
     node = __builtin_amdgcn_readfirstlane(node);
-    const float4* np = reinterpret_cast<const float4*>(S.nodes + node);
-    const float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5];
-    const int4 ch = reinterpret_cast<const int4*>(np)[6];
+    float4 lx, hx, ly, hy, lz, hz;
+    int4 ch;
+    if (kScalar) {
+      const UniformPtr np{S.nodes + node};
+      lx = load4(np, 0); hx = load4(np, 1); ly = load4(np, 2);
+      hy = load4(np, 3); lz = load4(np, 4); hz = load4(np, 5);
+      ch = load_int4_uniform(np.p, 6);
+    } else {
+      const float4* np = reinterpret_cast<const float4*>(S.nodes + node);
+      lx = np[0]; hx = np[1]; ly = np[2]; hy = np[3]; lz = np[4]; hz = np[5];
+      ch = reinterpret_cast<const int4*>(np)[6];
+    }
     const float limit = h.t * kCullSlack;
     float te[4];
     te[0] = slab_enter<kParallel>(rp, lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, limit);
@@ -231,7 +264,11 @@ __device__ __forceinline__ void closest_hit_packet_impl(const SceneView& S, cons
       const bool mine = active && te[s] < INFINITY;
       if (c == kChildEmpty || __ballot(mine) == 0ull) continue;
       if (child_is_leaf(c)) {
-        if (active) test_leaf(S.tris, c, rp, h);
+        // uniform triangle loads; inactive lanes test a dummy ray whose result is discarded
+        if (kScalar)
+          test_leaf(UniformPtr{S.tris}, c, rp, h);
+        else
+          test_leaf(VaryingPtr{S.tris}, c, rp, h);
       } else {
         // order key: the entry distance of the first lane that enters the child
         const uint64_t m = __ballot(mine);
@@ -269,6 +306,7 @@ __device__ __forceinline__ void closest_hit_packet_impl(const SceneView& S, cons
   }
 }
 
+template <bool kScalar = false>
 __device__ __forceinline__ void closest_hit_packet(const SceneView& S, const RayPre& rp, bool active, int* wstack,
                                                    Hit& h) {
   h.t = rp.tmax;
@@ -277,9 +315,9 @@ __device__ __forceinline__ void closest_hit_packet(const SceneView& S, const Ray
   h.id = 0xffffffffu;
   if (S.node_count == 0) return;
   if (__ballot(active && rp.par != 0) != 0ull)
-    closest_hit_packet_impl<true>(S, rp, active, wstack, h);
+    closest_hit_packet_impl<true, kScalar>(S, rp, active, wstack, h);
   else
-    closest_hit_packet_impl<false>(S, rp, active, wstack, h);
+    closest_hit_packet_impl<false, kScalar>(S, rp, active, wstack, h);
 }
 
 __global__ __launch_bounds__(kTraceThreads) void k_trace_samples_packet(SceneView S, EyeLaunch L, uint64_t total,
@@ -319,7 +357,7 @@ __device__ __forceinline__ uint32_t unit_count(const UnitLaunch& U, uint64_t u, 
   return (uint32_t)U.spp;
 }
 
-template <bool kStats, bool kSamples>
+template <bool kStats, bool kSamples, bool kScalar>
 __global__ __launch_bounds__(kTraceThreads) void k_render_units(SceneView S, UnitLaunch U, int wstride) {
   extern __shared__ int wave_stacks[];
   const int lane = threadIdx.x & 63;
@@ -359,7 +397,7 @@ __global__ __launch_bounds__(kTraceThreads) void k_render_units(SceneView S, Uni
         }
         const RayPre rp = precompute_ray(ray);
         Hit h;
-        closest_hit_packet(S, rp, active, wst, h);
+        closest_hit_packet<kScalar>(S, rp, active, wst, h);
         if (active) {
           const V3 c = shade(S, h, ray.dx, ray.dy, ray.dz);
           r = r + c.x * inv;
@@ -547,6 +585,22 @@ size_t stack_bytes(const DeviceScene& s) {
 
 }  // namespace
 
+static int g_trace_mode = kTracePacket;
+
+template <bool kScalar>
+static void launch_units_variant(const UnitLaunch& U, const SceneView& v, int blocks, size_t lds, int wstride,
+                                 hipStream_t stream) {
+  const dim3 grid(blocks), block(kTraceThreads);
+  if (U.stats && U.samples)
+    hipLaunchKernelGGL((k_render_units<true, true, kScalar>), grid, block, lds, stream, v, U, wstride);
+  else if (U.stats)
+    hipLaunchKernelGGL((k_render_units<true, false, kScalar>), grid, block, lds, stream, v, U, wstride);
+  else if (U.samples)
+    hipLaunchKernelGGL((k_render_units<false, true, kScalar>), grid, block, lds, stream, v, U, wstride);
+  else
+    hipLaunchKernelGGL((k_render_units<false, false, kScalar>), grid, block, lds, stream, v, U, wstride);
+}
+
 bool launch_render_units(const DeviceScene& scene, const UnitLaunch& U, int blocks, hipStream_t stream,
                          std::string* err) {
   const uint64_t units = (uint64_t)U.pose_count * (uint64_t)U.omm_count;
@@ -576,18 +630,12 @@ bool launch_render_units(const DeviceScene& scene, const UnitLaunch& U, int bloc
   if (waves > need) blocks = (int)((need + (kTraceThreads / 64) - 1) / (kTraceThreads / 64));
   hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, stream, U.queue, 1u, 0u);
   const SceneView v = view_of(scene);
-  if (U.stats && U.samples)
-    hipLaunchKernelGGL((k_render_units<true, true>), dim3(blocks), dim3(kTraceThreads), lds, stream, v, U, wstride);
-  else if (U.stats)
-    hipLaunchKernelGGL((k_render_units<true, false>), dim3(blocks), dim3(kTraceThreads), lds, stream, v, U, wstride);
-  else if (U.samples)
-    hipLaunchKernelGGL((k_render_units<false, true>), dim3(blocks), dim3(kTraceThreads), lds, stream, v, U, wstride);
+  if (trace_mode() == kTracePacketScalar)
+    launch_units_variant<true>(U, v, blocks, lds, wstride, stream);
   else
-    hipLaunchKernelGGL((k_render_units<false, false>), dim3(blocks), dim3(kTraceThreads), lds, stream, v, U, wstride);
+    launch_units_variant<false>(U, v, blocks, lds, wstride, stream);
   return check_launch("render_units", err);
 }
-
-static int g_trace_mode = kTracePacket;
 
 void set_trace_mode(int mode) { g_trace_mode = mode; }
 int trace_mode() { return g_trace_mode; }
@@ -596,7 +644,7 @@ bool launch_trace_samples(const DeviceScene& scene, const EyeLaunch& L, float4* 
                           std::string* err) {
   const uint64_t total = L.rays_per_pose * (uint64_t)L.pose_count;
   if (total == 0) return true;
-  if (g_trace_mode == kTracePacket) {
+  if (g_trace_mode != kTracePerRay) {
     const int wstride = scene.stack_bound > 0 ? scene.stack_bound : 1;
     const size_t lds = sizeof(int) * (size_t)wstride * (kTraceThreads / 64);
     const uint64_t blocks = (total + kTraceThreads - 1) / kTraceThreads;

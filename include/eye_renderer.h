@@ -200,7 +200,8 @@ size_t copySceneSortOrder(uint32_t* ids, uint32_t* codes, size_t count);
 int setSceneLeafSize(int leafSize);
 
 /* Traversal strategy of eye rays: 0 = one ray per lane, 1 = wave packets
- * (default).  Results are identical; only speed differs. */
+ * (default), 2 = wave packets reading nodes and triangles through the scalar
+ * unit.  Results are identical; only speed differs. */
 void setTraversalMode(int mode);
 /* Keep the per-sample records of renderFrame for getCurrentEyeSamples (off by
  * default: the packet renderer reduces samples in registers). */

@@ -110,10 +110,11 @@ def test_render_bit_exact_c1(lib, config1, oracle_c1):
     assert rel_close(col, ref["colour_seq"][0], 1e-5)
 
 
-@pytest.mark.parametrize("mode", [0, 1])
+@pytest.mark.parametrize("mode", [0, 1, 2])
 @pytest.mark.parametrize("spp", [7, 32, 64, 100])
 def test_traversal_modes_bit_exact(lib, config1, oracle_c1, mode, spp):
-    """Per-ray and wave-packet traversal give the oracle's samples exactly."""
+    """Per-ray and wave-packet traversal (vector or scalar node loads) give the
+    oracle's samples exactly."""
     lib.setTraversalMode(mode)
     try:
         omm = config1["eyes"][0][::3]
