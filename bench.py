@@ -112,6 +112,14 @@ def bench_batched(lib, er, scenes, tmp: str, repeats: int = 3) -> dict:
     er.set_sample_counts(lib, cfg["counts"])
     poses = cfg["poses"]
     rays = int(cfg["counts"].sum()) * len(poses)
+    # launch shape (units per queue grab, blocks, node-load variant) from the autotuner
+    t0 = time.perf_counter()
+    trials = er.autotune_np(lib, poses, repeats=1)
+    tune_s = time.perf_counter() - t0
+    tuning = er.render_tuning(lib)
+    default_ms = float(trials[(trials[:, 0] == 4) & (trials[:, 2] == 0)][:, 3].min())
+    log(f"autotune: {len(trials)} shapes in {tune_s:.1f} s -> {tuning}; sample ms best "
+        f"{trials[:, 3].min():.2f}, worst {trials[:, 3].max():.2f}")
     er.render_batch_np(lib, poses, want_rgb=False)  # warm-up (captures the graph)
     times = []
     t0 = time.perf_counter()
@@ -128,6 +136,9 @@ def bench_batched(lib, er, scenes, tmp: str, repeats: int = 3) -> dict:
         "gpu_ms_per_batch": gpu_ms,
         "wall_ms_per_batch": wall * 1e3,
         "eye_frames_per_s_gpu": len(poses) / (gpu_ms * 1e-3),
+        "autotune": {"shape": tuning, "candidates": int(len(trials)), "seconds": tune_s,
+                     "sample_ms_best": float(trials[:, 3].min()), "sample_ms_worst": float(trials[:, 3].max()),
+                     "sample_ms_grab4_vector_loads": default_ms},
     }
 
 

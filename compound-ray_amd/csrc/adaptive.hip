@@ -194,7 +194,7 @@ bool AdaptiveRenderer::render(const DeviceScene& scene, UnitLaunch U, const std:
   A.stats = d_stats_;
   A.samples = nullptr;
   A.queue = d_queue_;
-  if (!launch_render_units(scene, A, 0, stream, err)) return false;
+  if (!launch_render_units(scene, A, stream, err)) return false;
   const unsigned gb = (unsigned)((units + kThreads - 1) / kThreads);
   hipLaunchKernelGGL(k_features, dim3(gb), dim3(kThreads), 0, stream, U.omms, d_poses_, U.omm_count, (uint64_t)units,
                      d_col_a_, d_stats_, d_feat_);
@@ -209,7 +209,7 @@ bool AdaptiveRenderer::render(const DeviceScene& scene, UnitLaunch U, const std:
   B.sample_base = (uint32_t)cfg.pilot;
   B.colours = d_col_b_;
   B.stats = nullptr;
-  if (!launch_render_units(scene, B, 0, stream, err)) return false;
+  if (!launch_render_units(scene, B, stream, err)) return false;
   hipLaunchKernelGGL(k_combine, dim3(gb), dim3(kThreads), 0, stream, d_col_a_, d_col_b_, d_extra_, cfg.pilot,
                      (uint64_t)units);
   if (!ok_launch("combine", err)) return false;
@@ -282,7 +282,7 @@ bool AdaptiveRenderer::training_set(const DeviceScene& scene, UnitLaunch U, cons
   A.stats = d_stats_;
   A.samples = nullptr;
   A.queue = d_queue_;
-  if (!launch_render_units(scene, A, 0, stream, err)) return false;
+  if (!launch_render_units(scene, A, stream, err)) return false;
   const unsigned gb = (unsigned)((units + kThreads - 1) / kThreads);
   hipLaunchKernelGGL(k_features, dim3(gb), dim3(kThreads), 0, stream, U.omms, d_poses_, U.omm_count, (uint64_t)units,
                      d_col_a_, d_stats_, d_feat_);
@@ -292,7 +292,7 @@ bool AdaptiveRenderer::training_set(const DeviceScene& scene, UnitLaunch U, cons
   B.unit_counts = d_extra_;
   B.sample_base = (uint32_t)pilot;
   B.colours = d_col_b_;
-  if (!launch_render_units(scene, B, 0, stream, err)) return false;
+  if (!launch_render_units(scene, B, stream, err)) return false;
   hipLaunchKernelGGL(k_labels, dim3(gb), dim3(kThreads), 0, stream, d_stats_, (uint64_t)units,
                      1.0f / (error_target * error_target), d_pred_);
   if (!ok_launch("labels", err)) return false;

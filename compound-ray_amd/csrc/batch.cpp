@@ -1,7 +1,10 @@
 #include "batch.h"
 
 #include <algorithm>
+#include <cstdio>
 #include <cstring>
+#include <fstream>
+#include <sstream>
 
 namespace eye {
 
@@ -27,9 +30,90 @@ bool same_launch(const UnitLaunch& a, const UnitLaunch& b) {
          a.pose_base == b.pose_base && a.seed == b.seed && a.eye == b.eye && a.frame == b.frame && a.spp == b.spp &&
          a.omm_counts == b.omm_counts && a.unit_counts == b.unit_counts && a.sample_base == b.sample_base &&
          a.order == b.order && a.colours == b.colours && a.stats == b.stats && a.samples == b.samples &&
-         a.sample_offsets == b.sample_offsets && a.queue == b.queue && a.units_per_grab == b.units_per_grab;
+         a.sample_offsets == b.sample_offsets && a.queue == b.queue && a.units_per_grab == b.units_per_grab &&
+         a.blocks == b.blocks && a.scalar_loads == b.scalar_loads;
 }
 }  // namespace
+
+std::vector<LaunchShape> LaunchTuner::candidates(int capacity) {
+  std::vector<LaunchShape> out;
+  const int cap = std::max(capacity, 1);
+  const int block_opts[3] = {std::max(cap / 2, 1), cap, 2 * cap};
+  for (int scalar = 0; scalar < 2; ++scalar)
+    for (int b : block_opts)
+      for (int g : {1, 2, 4, 8, 16}) {
+        LaunchShape sh;
+        sh.units_per_grab = g;
+        sh.blocks = b;
+        sh.scalar_loads = scalar;
+        out.push_back(sh);
+      }
+  return out;
+}
+
+std::string LaunchTuner::key_of(const DeviceScene& scene, const UnitLaunch& U, uint64_t samples_per_pose) {
+  char buf[160];
+  std::snprintf(buf, sizeof(buf), "tris%d_nodes%d_leaf%d_omm%d_spp%llu_stats%d_rec%d", scene.tri_count,
+                scene.node_count, scene.leaf_max, U.omm_count, (unsigned long long)samples_per_pose,
+                U.stats ? 1 : 0, U.samples ? 1 : 0);
+  return buf;
+}
+
+bool LaunchTuner::find(const std::string& key, LaunchShape* shape) const {
+  for (size_t i = keys_.size(); i-- > 0;)
+    if (keys_[i] == key) {
+      *shape = shapes_[i];
+      return true;
+    }
+  return false;
+}
+
+void LaunchTuner::put(const std::string& key, const LaunchShape& shape, double ms) {
+  for (size_t i = 0; i < keys_.size(); ++i)
+    if (keys_[i] == key) {
+      shapes_[i] = shape;
+      ms_[i] = ms;
+      return;
+    }
+  keys_.push_back(key);
+  shapes_.push_back(shape);
+  ms_.push_back(ms);
+}
+
+bool LaunchTuner::load(const std::string& path, std::string* err) {
+  std::ifstream f(path);
+  if (!f) return true;  // no file yet: nothing cached
+  std::string line;
+  int lineno = 0;
+  while (std::getline(f, line)) {
+    ++lineno;
+    if (line.empty() || line[0] == '#') continue;
+    std::istringstream in(line);
+    std::string key;
+    LaunchShape sh;
+    double ms = 0.0;
+    if (!(in >> key >> sh.units_per_grab >> sh.blocks >> sh.scalar_loads >> ms) || sh.units_per_grab < 1 ||
+        sh.blocks < 0 || (sh.scalar_loads != 0 && sh.scalar_loads != 1)) {
+      if (err) *err = "autotune cache " + path + ": bad entry on line " + std::to_string(lineno);
+      return false;
+    }
+    put(key, sh, ms);
+  }
+  return true;
+}
+
+bool LaunchTuner::save(const std::string& path, std::string* err) const {
+  std::ofstream f(path, std::ios::trunc);
+  if (!f) {
+    if (err) *err = "autotune cache " + path + ": cannot write";
+    return false;
+  }
+  f << "# key units_per_grab blocks scalar_loads ms\n";
+  for (size_t i = 0; i < keys_.size(); ++i)
+    f << keys_[i] << ' ' << shapes_[i].units_per_grab << ' ' << shapes_[i].blocks << ' ' << shapes_[i].scalar_loads
+      << ' ' << ms_[i] << '\n';
+  return bool(f);
+}
 
 void BatchRenderer::release() {
   if (exec_) (void)hipGraphExecDestroy(exec_);
@@ -46,11 +130,11 @@ void BatchRenderer::release() {
   d_queue_ = nullptr;
   ev0_ = ev1_ = nullptr;
   poses_cap_ = colours_cap_ = 0;
-  g_blocks_ = -1;
+  g_valid_ = false;
   g_scene_ = nullptr;
 }
 
-bool BatchRenderer::build_graph(const DeviceScene& scene, const UnitLaunch& U, int blocks, hipStream_t stream,
+bool BatchRenderer::build_graph(const DeviceScene& scene, const UnitLaunch& U, hipStream_t stream,
                                 std::string* err) {
   if (exec_) (void)hipGraphExecDestroy(exec_);
   if (graph_) (void)hipGraphDestroy(graph_);
@@ -62,7 +146,7 @@ bool BatchRenderer::build_graph(const DeviceScene& scene, const UnitLaunch& U, i
     return false;
   }
   std::string lerr;
-  const bool ok = launch_render_units(scene, U, blocks, stream, &lerr);
+  const bool ok = launch_render_units(scene, U, stream, &lerr);
   e = hipStreamEndCapture(stream, &graph_);
   if (!ok || e != hipSuccess) {
     if (err) *err = !ok ? lerr : std::string("batch: end capture: ") + hipGetErrorString(e);
@@ -76,22 +160,20 @@ bool BatchRenderer::build_graph(const DeviceScene& scene, const UnitLaunch& U, i
     return false;
   }
   g_launch_ = U;
-  g_blocks_ = blocks;
+  g_valid_ = true;
   g_scene_ = scene.nodes;
   return true;
 }
 
-bool BatchRenderer::render(const DeviceScene& scene, UnitLaunch U, const std::vector<Pose>& poses, float* rgb,
-                           hipStream_t stream, double* gpu_ms, std::string* err) {
+bool BatchRenderer::prepare(const UnitLaunch& U, const std::vector<Pose>& poses, hipStream_t stream, bool* realloc,
+                            std::string* err) {
   const size_t np = poses.size();
-  *gpu_ms = 0.0;
-  if (np == 0) return true;
   if (!ev0_ && (hipEventCreate(&ev0_) != hipSuccess || hipEventCreate(&ev1_) != hipSuccess)) {
     if (err) *err = "batch: event creation failed";
     return false;
   }
   const size_t units = np * (size_t)U.omm_count;
-  const bool realloc = poses_cap_ < np || colours_cap_ < units;
+  *realloc = poses_cap_ < np || colours_cap_ < units;
   if (!grow(&d_poses_, &poses_cap_, np, err)) return false;
   if (!grow(&d_colours_, &colours_cap_, units, err)) return false;
   if (!d_queue_ && hipMalloc(&d_queue_, sizeof(uint32_t) * 4) != hipSuccess) {
@@ -102,15 +184,70 @@ bool BatchRenderer::render(const DeviceScene& scene, UnitLaunch U, const std::ve
     if (err) *err = "batch: pose upload failed";
     return false;
   }
+  return true;
+}
+
+bool BatchRenderer::tune(const DeviceScene& scene, UnitLaunch U, const std::vector<Pose>& poses, int repeats,
+                         hipStream_t stream, std::vector<TuneTrial>* trials, LaunchShape* best, std::string* err) {
+  trials->clear();
+  if (poses.empty() || U.omm_count == 0) {
+    if (err) *err = "autotune: empty batch";
+    return false;
+  }
+  bool realloc = false;
+  if (!prepare(U, poses, stream, &realloc, err)) return false;
+  U.poses = d_poses_;
+  U.pose_count = (int)poses.size();
+  U.pose_base = 0;
+  U.colours = d_colours_;
+  U.queue = d_queue_;
+  U.stats = nullptr;
+  U.samples = nullptr;
+  const int capacity = render_units_capacity(scene, U);
+  double best_ms = 1e300;
+  for (const LaunchShape& sh : LaunchTuner::candidates(capacity)) {
+    U.units_per_grab = sh.units_per_grab;
+    U.blocks = sh.blocks;
+    U.scalar_loads = sh.scalar_loads;
+    if (!launch_render_units(scene, U, stream, err)) return false;  // untimed: caches, clocks
+    double ms_min = 1e300;
+    for (int r = 0; r < std::max(repeats, 1); ++r) {
+      (void)hipEventRecord(ev0_, stream);
+      if (!launch_render_units(scene, U, stream, err)) return false;
+      (void)hipEventRecord(ev1_, stream);
+      if (hipEventSynchronize(ev1_) != hipSuccess) {
+        if (err) *err = "autotune: render failed";
+        return false;
+      }
+      float ms = 0.0f;
+      (void)hipEventElapsedTime(&ms, ev0_, ev1_);
+      ms_min = std::min(ms_min, (double)ms);
+    }
+    trials->push_back(TuneTrial{sh, ms_min});
+    if (ms_min < best_ms) {
+      best_ms = ms_min;
+      *best = sh;
+    }
+  }
+  return true;
+}
+
+bool BatchRenderer::render(const DeviceScene& scene, UnitLaunch U, const std::vector<Pose>& poses, float* rgb,
+                           hipStream_t stream, double* gpu_ms, std::string* err) {
+  const size_t np = poses.size();
+  *gpu_ms = 0.0;
+  if (np == 0) return true;
+  const size_t units = np * (size_t)U.omm_count;
+  bool realloc = false;
+  if (!prepare(U, poses, stream, &realloc, err)) return false;
   U.poses = d_poses_;
   U.pose_count = (int)np;
   U.pose_base = 0;
   U.colours = d_colours_;
   U.queue = d_queue_;
-  const int blocks = 0;
   if (use_graph_) {
-    if (realloc || !exec_ || g_scene_ != scene.nodes || !same_launch(U, g_launch_) || g_blocks_ != blocks)
-      if (!build_graph(scene, U, blocks, stream, err)) return false;
+    if (realloc || !exec_ || !g_valid_ || g_scene_ != scene.nodes || !same_launch(U, g_launch_))
+      if (!build_graph(scene, U, stream, err)) return false;
     (void)hipEventRecord(ev0_, stream);
     if (hipGraphLaunch(exec_, stream) != hipSuccess) {
       if (err) *err = "batch: graph launch failed";
@@ -118,7 +255,7 @@ bool BatchRenderer::render(const DeviceScene& scene, UnitLaunch U, const std::ve
     }
   } else {
     (void)hipEventRecord(ev0_, stream);
-    if (!launch_render_units(scene, U, blocks, stream, err)) return false;
+    if (!launch_render_units(scene, U, stream, err)) return false;
   }
   (void)hipEventRecord(ev1_, stream);
   if (hipEventSynchronize(ev1_) != hipSuccess) {

@@ -102,6 +102,9 @@ struct State {
   uint32_t* d_queue = nullptr;   // work counter of the unit renderer
   int units_per_grab = 4;
   int unit_blocks = 0;           // 0: automatic
+  bool autotune = false;         // tune renderPoseBatch's launch shape per workload
+  LaunchTuner tuner;
+  std::string tune_cache;        // file the tuner's entries are saved to (empty: none)
   std::string name_buf;
 };
 
@@ -263,6 +266,8 @@ UnitLaunch unit_launch_of(EyeState& e, const Pose* d_poses, int pose_count) {
   U.colours = e.d_colours;
   U.queue = s.d_queue;
   U.units_per_grab = s.units_per_grab;
+  U.blocks = s.unit_blocks;
+  U.scalar_loads = trace_mode() == kTracePacketScalar ? 1 : 0;
   U.order = e.d_order;
   return U;
 }
@@ -307,7 +312,7 @@ double render_compound(EyeState& e, const Cam& c) {
       U.samples = e.d_samples;
       U.sample_offsets = e.d_sample_offsets;
     }
-    if (!launch_render_units(s.scene, U, s.unit_blocks, s.stream, &err)) throw std::runtime_error(err);
+    if (!launch_render_units(s.scene, U, s.stream, &err)) throw std::runtime_error(err);
     hip_check(hipEventRecord(s.ev[1], s.stream), "event");
   } else {
     if (!launch_trace_samples(s.scene, L, e.d_samples, s.stream, &err)) throw std::runtime_error(err);
@@ -916,16 +921,10 @@ int traceRays(const float* rays, size_t count, int32_t* primIds, float* tbb) {
   ABI_GUARD_END(return -1)
 }
 
-double renderPoseBatch(const float* poses, size_t poseCount, float* rgbOut) {
-  ABI_GUARD_BEGIN
-  State& s = st();
-  EyeState* e = current_eye();
-  if (!s.loaded || !e) throw std::runtime_error("renderPoseBatch: no compound eye selected");
-  if (poseCount == 0) return 0.0;
-  if (!poses) throw std::runtime_error("renderPoseBatch: null poses");
-  upload_eye(*e);
-  std::vector<Pose> ps(poseCount);
-  for (size_t k = 0; k < poseCount; ++k) {
+// Poses of a batch from (x, y, z, rx, ry, rz) rows, posed as setCameraPose does.
+static std::vector<Pose> poses_from6(const float* poses, size_t n) {
+  std::vector<Pose> ps(n);
+  for (size_t k = 0; k < n; ++k) {
     Pose& p = ps[k];
     const float* q = poses + 6 * k;
     pose_reset(&p);
@@ -934,12 +933,108 @@ double renderPoseBatch(const float* poses, size_t poseCount, float* rgbOut) {
     pose_rotate_around(&p, q[5], 0.0f, 0.0f, 1.0f);
     pose_move(&p, q[0], q[1], q[2]);
   }
+  return ps;
+}
+
+// Evenly spaced poses of a batch holding about 2^20 units (all of a smaller
+// batch): the sample the tuner times.
+static std::vector<Pose> tuning_sample(const std::vector<Pose>& ps, size_t omm_count) {
+  const size_t want = std::max<size_t>(1, ((size_t)1 << 20) / std::max<size_t>(omm_count, 1));
+  if (ps.size() <= want) return ps;
+  std::vector<Pose> out(want);
+  for (size_t k = 0; k < want; ++k) out[k] = ps[k * ps.size() / want];
+  return out;
+}
+
+static LaunchShape tune_shape(EyeState& e, UnitLaunch U, const std::vector<Pose>& ps, int repeats,
+                              std::vector<TuneTrial>* trials) {
+  State& s = st();
+  std::string err;
+  LaunchShape best;
+  if (!s.batch.tune(s.scene, U, tuning_sample(ps, e.omms.size()), repeats, s.stream, trials, &best, &err))
+    throw std::runtime_error(err);
+  double best_ms = 0.0;
+  for (const TuneTrial& t : *trials)
+    if (t.shape.units_per_grab == best.units_per_grab && t.shape.blocks == best.blocks &&
+        t.shape.scalar_loads == best.scalar_loads)
+      best_ms = t.ms;
+  s.tuner.put(LaunchTuner::key_of(s.scene, U, e.total_samples()), best, best_ms);
+  if (!s.tune_cache.empty() && !s.tuner.save(s.tune_cache, &err)) throw std::runtime_error(err);
+  return best;
+}
+
+double renderPoseBatch(const float* poses, size_t poseCount, float* rgbOut) {
+  ABI_GUARD_BEGIN
+  State& s = st();
+  EyeState* e = current_eye();
+  if (!s.loaded || !e) throw std::runtime_error("renderPoseBatch: no compound eye selected");
+  if (poseCount == 0) return 0.0;
+  if (!poses) throw std::runtime_error("renderPoseBatch: null poses");
+  upload_eye(*e);
+  const std::vector<Pose> ps = poses_from6(poses, poseCount);
   UnitLaunch U = unit_launch_of(*e, nullptr, (int)poseCount);
+  if (s.autotune) {
+    LaunchShape shape;
+    if (!s.tuner.find(LaunchTuner::key_of(s.scene, U, e->total_samples()), &shape)) {
+      std::vector<TuneTrial> trials;
+      shape = tune_shape(*e, U, ps, 1, &trials);
+    }
+    U.units_per_grab = shape.units_per_grab;
+    U.blocks = shape.blocks;
+    U.scalar_loads = shape.scalar_loads;
+  }
   std::string err;
   double ms = -1.0;
   if (!s.batch.render(s.scene, U, ps, rgbOut, s.stream, &ms, &err)) throw std::runtime_error(err);
   return ms;
   ABI_GUARD_END(return -1.0)
+}
+
+void setBatchAutotune(int enabled) { st().autotune = enabled != 0; }
+
+int setAutotuneCacheFile(const char* path) {
+  ABI_GUARD_BEGIN
+  State& s = st();
+  s.tune_cache = path ? path : "";
+  std::string err;
+  if (!s.tune_cache.empty() && !s.tuner.load(s.tune_cache, &err)) throw std::runtime_error(err);
+  return (int)s.tuner.size();
+  ABI_GUARD_END(return -1)
+}
+
+int autotuneRenderBatch(const float* poses, size_t poseCount, int repeats, float* trials, size_t maxTrials) {
+  ABI_GUARD_BEGIN
+  State& s = st();
+  EyeState* e = current_eye();
+  if (!s.loaded || !e) throw std::runtime_error("autotuneRenderBatch: no compound eye selected");
+  if (!poses || poseCount == 0) throw std::runtime_error("autotuneRenderBatch: no poses");
+  upload_eye(*e);
+  const std::vector<Pose> ps = poses_from6(poses, poseCount);
+  UnitLaunch U = unit_launch_of(*e, nullptr, (int)poseCount);
+  std::vector<TuneTrial> tr;
+  const LaunchShape best = tune_shape(*e, U, ps, std::max(repeats, 1), &tr);
+  // the winner becomes the shape of every later unit launch
+  s.units_per_grab = best.units_per_grab;
+  s.unit_blocks = best.blocks;
+  set_trace_mode(best.scalar_loads ? kTracePacketScalar : kTracePacket);
+  if (trials)
+    for (size_t i = 0; i < tr.size() && i < maxTrials; ++i) {
+      trials[4 * i + 0] = (float)tr[i].shape.units_per_grab;
+      trials[4 * i + 1] = (float)tr[i].shape.blocks;
+      trials[4 * i + 2] = (float)tr[i].shape.scalar_loads;
+      trials[4 * i + 3] = (float)tr[i].ms;
+    }
+  return (int)tr.size();
+  ABI_GUARD_END(return -1)
+}
+
+int getRenderTuning(int* shape) {
+  if (!shape) return -1;
+  State& s = st();
+  shape[0] = s.units_per_grab;
+  shape[1] = s.unit_blocks;
+  shape[2] = trace_mode();
+  return 0;
 }
 
 size_t mlpParamCount(void) { return (size_t)kMlpParams; }
@@ -1024,19 +1119,6 @@ int mlpTrain(const float* features, const float* targets, size_t rows, int epoch
   ABI_GUARD_END(return -1)
 }
 
-static std::vector<Pose> poses_from6(const float* poses, size_t n) {
-  std::vector<Pose> ps(n);
-  for (size_t k = 0; k < n; ++k) {
-    Pose& p = ps[k];
-    const float* q = poses + 6 * k;
-    pose_reset(&p);
-    pose_rotate_around(&p, q[3], 1.0f, 0.0f, 0.0f);
-    pose_rotate_around(&p, q[4], 0.0f, 1.0f, 0.0f);
-    pose_rotate_around(&p, q[5], 0.0f, 0.0f, 1.0f);
-    pose_move(&p, q[0], q[1], q[2]);
-  }
-  return ps;
-}
 
 int64_t adaptiveTrainingSet(const float* poses, size_t poseCount, const uint8_t* unitMask, int pilot, int labelSpp,
                             float errorTarget, float* features, float* labels, size_t maxRows) {

@@ -49,13 +49,19 @@ struct UnitLaunch {
   float4* samples = nullptr;            // nullable: per-sample (r, g, b, t or -1)
   const uint64_t* sample_offsets = nullptr;  // first sample record of each unit
   uint32_t* queue = nullptr;            // device work counter
-  int units_per_grab = 4;
+  // launch shape (results do not depend on it)
+  int units_per_grab = 4;               // consecutive units a wave takes from the queue
+  int blocks = 0;                       // blocks of 256 threads; 0: every resident slot once
+  int scalar_loads = 0;                 // read nodes/triangles through the scalar unit
 };
 
-// Launches the unit renderer on `blocks` blocks of 256 threads (0: one block
-// per (CU x occupancy) slot).  The queue counter is cleared by a kernel first.
-bool launch_render_units(const DeviceScene& scene, const UnitLaunch& U, int blocks, hipStream_t stream,
-                         std::string* err);
+// Launches the unit renderer with U's launch shape.  The queue counter is
+// cleared by a kernel first.
+bool launch_render_units(const DeviceScene& scene, const UnitLaunch& U, hipStream_t stream, std::string* err);
+
+// Blocks of the unit renderer that fit on the device at once (CUs x resident
+// blocks per CU for this scene's stack size and U's kernel variant).
+int render_units_capacity(const DeviceScene& scene, const UnitLaunch& U);
 
 // Traversal of eye rays: one ray per lane with its own LDS stack, or the
 // wave-packet traversal (coherent rays of one ommatidium share the walk),

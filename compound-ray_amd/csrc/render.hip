@@ -601,8 +601,41 @@ static void launch_units_variant(const UnitLaunch& U, const SceneView& v, int bl
     hipLaunchKernelGGL((k_render_units<false, false, kScalar>), grid, block, lds, stream, v, U, wstride);
 }
 
-bool launch_render_units(const DeviceScene& scene, const UnitLaunch& U, int blocks, hipStream_t stream,
-                         std::string* err) {
+template <bool kScalar>
+static const void* units_kernel_variant(const UnitLaunch& U) {
+  if (U.stats && U.samples) return reinterpret_cast<const void*>(&k_render_units<true, true, kScalar>);
+  if (U.stats) return reinterpret_cast<const void*>(&k_render_units<true, false, kScalar>);
+  if (U.samples) return reinterpret_cast<const void*>(&k_render_units<false, true, kScalar>);
+  return reinterpret_cast<const void*>(&k_render_units<false, false, kScalar>);
+}
+
+static int device_cus() {
+  static int cus = 0;
+  if (cus == 0) {
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+      cus = 256;
+  }
+  return cus;
+}
+
+static size_t units_lds_bytes(const DeviceScene& scene) {
+  const int wstride = scene.stack_bound > 0 ? scene.stack_bound : 1;
+  return sizeof(int) * (size_t)wstride * (kTraceThreads / 64);
+}
+
+int render_units_capacity(const DeviceScene& scene, const UnitLaunch& U) {
+  const void* fn = U.scalar_loads ? units_kernel_variant<true>(U) : units_kernel_variant<false>(U);
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kTraceThreads, units_lds_bytes(scene)) !=
+          hipSuccess ||
+      per_cu <= 0)
+    per_cu = 4;
+  return device_cus() * per_cu;
+}
+
+bool launch_render_units(const DeviceScene& scene, const UnitLaunch& U, hipStream_t stream, std::string* err) {
   const uint64_t units = (uint64_t)U.pose_count * (uint64_t)U.omm_count;
   if (units == 0) return true;
   if (units > 0xffffffffull - 65536ull) {
@@ -614,23 +647,14 @@ bool launch_render_units(const DeviceScene& scene, const UnitLaunch& U, int bloc
     return false;
   }
   const int wstride = scene.stack_bound > 0 ? scene.stack_bound : 1;
-  const size_t lds = sizeof(int) * (size_t)wstride * (kTraceThreads / 64);
-  if (blocks <= 0) {
-    static int cus = 0;
-    if (cus == 0) {
-      int dev = 0;
-      (void)hipGetDevice(&dev);
-      if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
-        cus = 256;
-    }
-    blocks = cus * 8;
-  }
+  const size_t lds = units_lds_bytes(scene);
+  int blocks = U.blocks > 0 ? U.blocks : render_units_capacity(scene, U);
   const uint64_t need = (units + (uint64_t)U.units_per_grab - 1) / (uint64_t)U.units_per_grab;
   const uint64_t waves = (uint64_t)blocks * (kTraceThreads / 64);
   if (waves > need) blocks = (int)((need + (kTraceThreads / 64) - 1) / (kTraceThreads / 64));
   hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, stream, U.queue, 1u, 0u);
   const SceneView v = view_of(scene);
-  if (trace_mode() == kTracePacketScalar)
+  if (U.scalar_loads)
     launch_units_variant<true>(U, v, blocks, lds, wstride, stream);
   else
     launch_units_variant<false>(U, v, blocks, lds, wstride, stream);

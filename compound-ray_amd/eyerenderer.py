@@ -135,6 +135,13 @@ def configure_extensions(lib):
     lib.setTraversalMode.argtypes = [c_int]
     lib.setSampleRecording.argtypes = [c_bool]
     lib.setRenderTuning.argtypes = [c_int, c_int]
+    lib.getRenderTuning.argtypes = [i32p]
+    lib.getRenderTuning.restype = c_int
+    lib.setBatchAutotune.argtypes = [c_int]
+    lib.setAutotuneCacheFile.argtypes = [c_char_p]
+    lib.setAutotuneCacheFile.restype = c_int
+    lib.autotuneRenderBatch.argtypes = [f32p, c_size_t, c_int, c_void_p, c_size_t]
+    lib.autotuneRenderBatch.restype = c_int
     lib.runCopyProbe.argtypes = [c_size_t, c_int]
     lib.runCopyProbe.restype = c_double
     lib.mlpParamCount.restype = c_size_t
@@ -303,6 +310,23 @@ def render_batch_np(lib, poses6: np.ndarray, want_rgb: bool = True):
     if ms < 0:
         raise RuntimeError(last_error(lib))
     return ms, out
+
+
+def render_tuning(lib) -> dict:
+    v = np.zeros(3, np.int32)
+    lib.getRenderTuning(v)
+    return {"units_per_grab": int(v[0]), "blocks": int(v[1]), "trace_mode": int(v[2])}
+
+
+def autotune_np(lib, poses6: np.ndarray, repeats: int = 1) -> np.ndarray:
+    """Times the candidate launch shapes on these poses and installs the
+    fastest; returns rows (units_per_grab, blocks, scalar_loads, ms)."""
+    p = np.ascontiguousarray(poses6, dtype=np.float32).reshape(-1, 6)
+    rows = np.zeros((64, 4), np.float32)
+    n = lib.autotuneRenderBatch(p, p.shape[0], repeats, rows.ctypes.data_as(c_void_p), rows.shape[0])
+    if n < 0:
+        raise RuntimeError(last_error(lib))
+    return rows[:n]
 
 
 def training_set_np(lib, poses6, unit_mask=None, pilot=8, label_spp=1024, error_target=0.01):

@@ -207,8 +207,23 @@ void setTraversalMode(int mode);
  * default: the packet renderer reduces samples in registers). */
 void setSampleRecording(bool enabled);
 /* Launch shape of the persistent unit renderer: ommatidia taken from the work
- * queue per grab, and blocks of 256 threads (0 = 8 per compute unit). */
+ * queue per grab, and blocks of 256 threads (0 = every resident slot once). */
 void setRenderTuning(int unitsPerGrab, int blocks);
+/* Writes the current launch shape: units per grab, blocks, traversal mode. */
+int getRenderTuning(int* shape3);
+/* Launch-shape autotuning of renderPoseBatch (no reference counterpart; the
+ * reference has a single launch shape).  When enabled, the first batch of each
+ * workload (scene, eye, sample counts) times the candidate shapes on a sample
+ * of its poses and later batches reuse the winner. */
+void setBatchAutotune(int enabled);
+/* Loads tuned shapes from a text file and saves later tunes there; returns
+ * the number of cached entries, -1 on a malformed file. */
+int setAutotuneCacheFile(const char* path);
+/* Times every candidate shape on (a sample of) these poses and installs the
+ * fastest as the shape of all later unit launches.  trials receives rows of
+ * 4 floats (units per grab, blocks, scalar loads, ms); returns the number of
+ * candidates, -1 on error. */
+int autotuneRenderBatch(const float* poses, size_t poseCount, int repeats, float* trials, size_t maxTrials);
 
 /* Bandwidth reference for profiling: copies `bytes` (rounded down to 16-byte
  * units) between two device buffers with a plain vector copy kernel

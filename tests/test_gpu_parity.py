@@ -214,6 +214,41 @@ def test_pose_batch(lib, config1, oracle_c1):
     assert np.array_equal(rgb, ref["colour_gpu_order"])
 
 
+def test_autotuned_batch_matches_oracle(lib, config1, oracle_c1, tmp_path):
+    """The autotuner times every candidate launch shape; the batch rendered with
+    the winning shape (and through the tuning cache) equals the oracle."""
+    omm = config1["eyes"][0][:300]
+    setup_eye(lib, config1, omm=omm, spp=16)
+    counts = scenes.log_uniform_counts(300, 8, 64, seed=3)
+    er.set_sample_counts(lib, counts)
+    poses6 = scenes.batch_poses(config1["scene"], 10, seed=8, extent=10.0)
+    poses12 = np.stack([oracle.pose12(p) for p in poses6])
+    ref = oracle_c1.render(omm, poses12, counts=counts, frame=lib.getCurrentEyeFrameIndex())["colour_gpu_order"]
+    try:
+        rows = er.autotune_np(lib, poses6, repeats=1)
+        assert rows.shape == (30, 4) and np.all(rows[:, 3] > 0)
+        best = rows[np.argmin(rows[:, 3])]
+        tun = er.render_tuning(lib)
+        assert tun["units_per_grab"] == int(best[0]) and tun["blocks"] == int(best[1])
+        assert tun["trace_mode"] == (2 if best[2] else 1)
+        _, rgb = er.render_batch_np(lib, poses6)
+        assert np.array_equal(rgb, ref)
+        # automatic tuning inside renderPoseBatch, saved to and reloaded from the cache file
+        cache = tmp_path / "tune.txt"
+        assert lib.setAutotuneCacheFile(str(cache).encode()) >= 0
+        lib.setBatchAutotune(1)
+        _, rgb2 = er.render_batch_np(lib, poses6)
+        assert np.array_equal(rgb2, ref)
+        lines = [ln for ln in cache.read_text().splitlines() if not ln.startswith("#")]
+        assert any(ln.split()[0].startswith("tris19602_") for ln in lines)
+        assert lib.setAutotuneCacheFile(str(cache).encode()) >= 1
+    finally:
+        lib.setBatchAutotune(0)
+        lib.setAutotuneCacheFile(None)
+        lib.setRenderTuning(4, 0)
+        lib.setTraversalMode(1)
+
+
 def test_spp_edge_values(lib, config1, oracle_c1):
     omm = config1["eyes"][0][:64]
     for spp in (1, 63, 64, 65, 129):
