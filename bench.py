@@ -15,6 +15,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import pathlib
 import platform
@@ -142,6 +143,163 @@ def bench_batched(lib, er, scenes, tmp: str, repeats: int = 3) -> dict:
     }
 
 
+def rms_abs(a, b) -> float:
+    return float(np.sqrt(np.mean((np.asarray(a, np.float64) - np.asarray(b, np.float64)) ** 2)))
+
+
+def bench_adaptive(lib, er, scenes, cfg, eyes, target: float = 0.01, n_test: int = 1024, n_train: int = 8192,
+                   train_units: int = 2_000_000, label_spp: int = 1024, n_ref: int = 16) -> dict:
+    """Learned adaptive sampling on the bee-scale scene and eyes: the sampler
+    network is trained on renders of held-out poses, then 1,024 test poses of
+    both eyes are rendered (a) adaptively and (b) uniformly at the smallest
+    rate reaching the same RMS colour error against a 4,096-spp render, in
+    one batched launch each; (c) uniformly through renderFrame, pose by pose
+    (the reference's way of rendering)."""
+    scene = cfg["scene"]
+    test = scenes.batch_poses(scene, n_test, seed=4242, extent=40.0)
+    train = scenes.batch_poses(scene, n_train, seed=777, extent=40.0)
+    ref_poses = test[:n_ref]
+    # 1. training set: pilot features + labels from label_spp further samples
+    t0 = time.perf_counter()
+    rng = np.random.default_rng(9)
+    feats, labels = [], []
+    chunk = 512
+    for i in eyes:
+        lib.gotoCamera(i)
+        n_omm = lib.getCurrentEyeOmmatidialCount()
+        frac = train_units / len(eyes) / (n_train * n_omm)
+        for c0 in range(0, n_train, chunk):
+            P = train[c0:c0 + chunk]
+            mask = (rng.random((len(P), n_omm)) < frac).astype(np.uint8)
+            lib.setCurrentEyeFrameIndex(100000 + c0)
+            f, lab = er.training_set_np(lib, P, unit_mask=mask, pilot=8, label_spp=label_spp, error_target=target)
+            feats.append(f)
+            labels.append(lab)
+    F = np.concatenate(feats)
+    L = np.concatenate(labels)
+    t_set = time.perf_counter() - t0
+    log(f"adaptive: training set {len(L)} rows in {t_set:.1f} s")
+    # 2. training on the GPU (BF16 MFMA forward/backward, Adam)
+    if lib.mlpInit(1) != 0:
+        raise RuntimeError(er.last_error(lib))
+    t0 = time.perf_counter()
+    losses = er.mlp_train_np(lib, F, L, epochs=6, batch=4096, lr=2e-3, seed=2)
+    t_train = time.perf_counter() - t0
+    log(f"adaptive: trained in {t_train:.1f} s, loss {losses[0]:.3f} -> {losses[-1]:.3f}")
+
+    def per_eye(fn):
+        out = []
+        for i in eyes:
+            lib.gotoCamera(i)
+            out.append(fn())
+        return out
+
+    # 3. 4,096-spp reference at the first n_ref test poses (independent frame)
+    def ref_render():
+        lib.setCurrentEyeSamplesPerOmmatidium(4096)
+        lib.setCurrentEyeFrameIndex(1 << 20)
+        return er.render_batch_np(lib, ref_poses)[1]
+    ref = per_eye(ref_render)
+
+    def uniform_err(spp):
+        def f():
+            lib.setCurrentEyeSamplesPerOmmatidium(spp)
+            lib.setCurrentEyeFrameIndex(0)
+            return er.render_batch_np(lib, ref_poses)[1]
+        return rms_abs(np.stack(per_eye(f)), np.stack(ref))
+
+    # uniform rate for the target: RMS error falls as 1/sqrt(spp)
+    e64 = uniform_err(64)
+    spp_u = max(1, int(math.ceil(64 * (e64 / target) ** 2)))
+    e_u = uniform_err(spp_u)
+    while e_u > target and spp_u < 4096:
+        spp_u = int(math.ceil(spp_u * 1.1))
+        e_u = uniform_err(spp_u)
+
+    # adaptive error on the reference poses, budget calibrated to the target
+    def adaptive_err(scale):
+        got, spp_sum, units = [], 0, 0
+        for i in eyes:
+            lib.gotoCamera(i)
+            lib.setCurrentEyeFrameIndex(0)
+            _, rgb, _, total = er.render_adaptive_np(lib, ref_poses, error_target=target, max_spp=1024,
+                                                     budget_scale=scale)
+            got.append(rgb)
+            spp_sum += total
+            units += rgb.shape[0] * rgb.shape[1]
+        return rms_abs(np.stack(got), np.stack(ref)), spp_sum / units
+
+    scale = 1.0
+    e_a, spp_a_ref = adaptive_err(scale)
+    for _ in range(4):
+        if e_a <= target:
+            break
+        scale *= (e_a / target) ** 2 * 1.05
+        e_a, spp_a_ref = adaptive_err(scale)
+
+    # 4. timed: 1,024 poses x both eyes, adaptive vs uniform batched vs renderFrame
+    def timed(fn, repeats=2):
+        fn()  # warm-up
+        t = time.perf_counter()
+        for _ in range(repeats):
+            fn()
+        return (time.perf_counter() - t) / repeats
+
+    total_a = [0]
+
+    def run_adaptive():
+        total_a[0] = 0
+        for i in eyes:
+            lib.gotoCamera(i)
+            lib.setCurrentEyeFrameIndex(0)
+            total_a[0] += er.render_adaptive_np(lib, test, error_target=target, max_spp=1024, budget_scale=scale,
+                                                want_rgb=False)[3]
+
+    def run_uniform():
+        for i in eyes:
+            lib.gotoCamera(i)
+            lib.setCurrentEyeSamplesPerOmmatidium(spp_u)
+            er.render_batch_np(lib, test, want_rgb=False)
+
+    n_plain = 128
+
+    def run_plain():
+        for i in eyes:
+            lib.gotoCamera(i)
+            lib.setCurrentEyeSamplesPerOmmatidium(spp_u)
+            for p in test[:n_plain]:
+                lib.setCameraPose(*[float(v) for v in p])
+                if lib.renderFrame() < 0:
+                    raise RuntimeError(er.last_error(lib))
+
+    frames = n_test * len(eyes)
+    t_a = timed(run_adaptive)
+    t_u = timed(run_uniform)
+    t_p = timed(run_plain, repeats=1)
+    n_omm_total = sum(per_eye(lambda: lib.getCurrentEyeOmmatidialCount()))
+    efs_a = frames / t_a
+    efs_u = frames / t_u
+    efs_p = n_plain * len(eyes) / t_p
+    return {
+        "workload": f"learned adaptive vs uniform at <= {target} RMS colour error: 2 eyes x 5,500 ommatidia x "
+                    f"{n_test} poses over the bee-scale scene",
+        "eye_frames_per_s_adaptive": efs_a,
+        "eye_frames_per_s_uniform_batched": efs_u,
+        "eye_frames_per_s_uniform_renderframe": efs_p,
+        "gain_vs_uniform_batched": efs_a / efs_u,
+        "gain_vs_uniform_renderframe": efs_a / efs_p,
+        "uniform_spp": spp_u,
+        "adaptive_mean_spp": total_a[0] / (n_test * n_omm_total),
+        "rms_abs_uniform": e_u,
+        "rms_abs_adaptive": e_a,
+        "rms_reference": f"{n_ref} test poses x both eyes, uniform 4,096-spp render on the GPU",
+        "budget_scale": scale,
+        "training": {"rows": int(len(L)), "label_spp": label_spp, "poses": n_train, "set_s": t_set,
+                     "train_s": t_train, "epochs": len(losses), "loss_first": float(losses[0]),
+                     "loss_last": float(losses[-1])},
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -151,6 +309,7 @@ def main() -> int:
     ap.add_argument("--profile", action="store_true",
                     help="render loop only (no 4096-spp error check, no CPU baseline): for profiler runs")
     ap.add_argument("--skip-batched", action="store_true", help="skip the 4,096-pose batched configuration")
+    ap.add_argument("--skip-adaptive", action="store_true", help="skip the learned adaptive sampling comparison")
     ap.add_argument("--trace-mode", type=int, default=1, choices=(0, 1, 2),
                     help="traversal: 0 per-ray, 1 wave packets (default), 2 wave packets with scalar node loads")
     ap.add_argument("--copy-probe", type=float, default=0.0,
@@ -277,6 +436,9 @@ def main() -> int:
         lib.gotoCamera(eyes[0])
         pose12 = er.pose_np(lib)
         out["cpu_baseline"] = cpu_baseline(cfg, spp, pose12)
+    if not args.profile and not args.skip_adaptive:
+        log("learned adaptive sampling ...")
+        out["learned_adaptive"] = bench_adaptive(lib, er, scenes, cfg, eyes)
     if not args.profile and not args.skip_batched:
         log("batched configuration ...")
         out["batched"] = bench_batched(lib, er, scenes, tmp)

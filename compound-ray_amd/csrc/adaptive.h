@@ -30,6 +30,8 @@ namespace eye {
 
 struct AdaptiveConfig {
   float error_target = 0.01f;
+  float label_error = 0.01f;   // error the network's labels were made for: its
+                               //   n_req scale by (label_error / error_target)^2
   int pilot = 8;
   int max_spp = 512;
   float budget_scale = 1.0f;   // multiplies every allocation (calibration knob)

@@ -172,8 +172,8 @@ bool AdaptiveRenderer::render(const DeviceScene& scene, UnitLaunch U, const std:
     if (err) *err = "adaptive: the sampler network is not initialised (mlpInit / mlpTrain)";
     return false;
   }
-  if (cfg.pilot < 1 || cfg.max_spp < cfg.pilot) {
-    if (err) *err = "adaptive: need 1 <= pilot <= max samples";
+  if (cfg.pilot < 1 || cfg.max_spp < cfg.pilot || !(cfg.error_target > 0.0f) || !(cfg.label_error > 0.0f)) {
+    if (err) *err = "adaptive: need 1 <= pilot <= max samples and positive error targets";
     return false;
   }
   if (!ensure(np, units, err)) return false;
@@ -202,7 +202,9 @@ bool AdaptiveRenderer::render(const DeviceScene& scene, UnitLaunch U, const std:
   if (!mlp.predict(d_feat_, (int64_t)units, d_pred_, stream, err)) return false;
   hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, stream, d_total_);
   hipLaunchKernelGGL(k_allocate, dim3((unsigned)np), dim3(kThreads), 0, stream, d_pred_, U.omm_count, cfg.pilot,
-                     cfg.max_spp, cfg.budget_scale, d_counts_, d_extra_, d_total_);
+                     cfg.max_spp, cfg.budget_scale * (cfg.label_error / cfg.error_target) *
+                                      (cfg.label_error / cfg.error_target),
+                     d_counts_, d_extra_, d_total_);
   if (!ok_launch("allocate", err)) return false;
   UnitLaunch B = A;
   B.unit_counts = d_extra_;

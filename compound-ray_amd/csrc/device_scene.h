@@ -8,6 +8,7 @@
 #include <string>
 
 #include "bvh_format.h"
+#include "eyecore.h"
 
 namespace eye {
 
@@ -19,7 +20,7 @@ struct DeviceScene {
   //   [3k+1] = (v1.xyz, 0), [3k+2] = (v2.xyz, 0)
   float4* tris = nullptr;
   // Shading data in sorted order, three float4 each:
-  //   [3k+0] = (c0.rgb, colour mode as uint bits), [3k+1] = (c1.rgb, 0), [3k+2] = (c2.rgb, 0)
+  //   [3k+0] = (c0.rgb, colour mode word as uint bits), [3k+1] = (c1.rgb, 0), [3k+2] = (c2.rgb, 0)
   float4* cols = nullptr;
   uint32_t* sorted_ids = nullptr;  // sorted position -> original triangle id
   uint32_t* codes = nullptr;       // sorted Morton codes
@@ -28,6 +29,11 @@ struct DeviceScene {
   int levels = 0;
   int leaf_max = 4;
   int sky_mode = 0;
+  // Base-colour textures (not owned: the scene loader's upload).  A textured
+  // triangle's cols hold (u0, v0, u1, mode) and (v1, u2, v2, 0).
+  const uint32_t* texels = nullptr;
+  const TexView* textures = nullptr;
+  int texture_count = 0;
   double build_ms = 0.0;
   float centroid_lo[3] = {0, 0, 0}, centroid_hi[3] = {0, 0, 0};
 };

@@ -80,6 +80,8 @@ struct State {
   float* d_pos = nullptr;
   float* d_col = nullptr;
   uint32_t* d_mode = nullptr;
+  uint32_t* d_texels = nullptr;
+  TexView* d_textures = nullptr;
   std::vector<Cam> cams;
   std::vector<EyeState> eyes;
   size_t current = 0;
@@ -103,6 +105,7 @@ struct State {
   int units_per_grab = 4;
   int unit_blocks = 0;           // 0: automatic
   bool autotune = false;         // tune renderPoseBatch's launch shape per workload
+  float label_error = 0.01f;     // error the sampler network's labels were made for
   LaunchTuner tuner;
   std::string tune_cache;        // file the tuner's entries are saved to (empty: none)
   std::string name_buf;
@@ -171,8 +174,12 @@ void free_scene() {
   if (s.d_pos) (void)hipFree(s.d_pos);
   if (s.d_col) (void)hipFree(s.d_col);
   if (s.d_mode) (void)hipFree(s.d_mode);
+  if (s.d_texels) (void)hipFree(s.d_texels);
+  if (s.d_textures) (void)hipFree(s.d_textures);
   s.d_pos = s.d_col = nullptr;
   s.d_mode = nullptr;
+  s.d_texels = nullptr;
+  s.d_textures = nullptr;
   for (EyeState& e : s.eyes) {
     if (e.d_omms) (void)hipFree(e.d_omms);
     if (e.d_offsets) (void)hipFree(e.d_offsets);
@@ -289,6 +296,9 @@ void build_scene_bvh() {
                           &err))
     throw std::runtime_error(err);
   s.scene.sky_mode = s.host.sky_mode;
+  s.scene.texels = s.d_texels;
+  s.scene.textures = s.d_textures;
+  s.scene.texture_count = (int)s.host.textures.size();
 }
 
 double render_compound(EyeState& e, const Cam& c) {
@@ -422,6 +432,18 @@ void loadGlTFscene(const char* filepath) {
     hip_check(hipMemcpy(s.d_pos, s.host.pos.data(), sizeof(float) * 9 * n, hipMemcpyHostToDevice), "upload");
     hip_check(hipMemcpy(s.d_col, s.host.col.data(), sizeof(float) * 9 * n, hipMemcpyHostToDevice), "upload");
     hip_check(hipMemcpy(s.d_mode, s.host.mode.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice), "upload");
+  }
+  if (!s.host.textures.empty()) {
+    std::vector<TexView> tv;
+    for (const TextureDesc& d : s.host.textures)
+      tv.push_back(TexView{d.width, d.height, d.wrap_s, d.wrap_t, d.filter, d.first});
+    hip_check(hipMalloc(&s.d_texels, sizeof(uint32_t) * s.host.texels.size()), "hipMalloc texels");
+    hip_check(hipMalloc(&s.d_textures, sizeof(TexView) * tv.size()), "hipMalloc textures");
+    hip_check(hipMemcpy(s.d_texels, s.host.texels.data(), sizeof(uint32_t) * s.host.texels.size(),
+                        hipMemcpyHostToDevice), "upload");
+    hip_check(hipMemcpy(s.d_textures, tv.data(), sizeof(TexView) * tv.size(), hipMemcpyHostToDevice), "upload");
+    s.host.texels.clear();
+    s.host.texels.shrink_to_fit();
   }
   build_scene_bvh();
   // release the host copies of the geometry; the device keeps them for rebuilds
@@ -992,6 +1014,15 @@ double renderPoseBatch(const float* poses, size_t poseCount, float* rgbOut) {
 
 void setBatchAutotune(int enabled) { st().autotune = enabled != 0; }
 
+int setAdaptiveLabelError(float labelError) {
+  if (!(labelError > 0.0f)) {
+    set_error("setAdaptiveLabelError: the error must be positive");
+    return -1;
+  }
+  st().label_error = labelError;
+  return 0;
+}
+
 int setAutotuneCacheFile(const char* path) {
   ABI_GUARD_BEGIN
   State& s = st();
@@ -1137,6 +1168,7 @@ int64_t adaptiveTrainingSet(const float* poses, size_t poseCount, const uint8_t*
   if (!s.adaptive.training_set(s.scene, U, poses_from6(poses, poseCount), mask, pilot, labelSpp, errorTarget, &f,
                                &l, s.stream, &err))
     throw std::runtime_error(err);
+  s.label_error = errorTarget;
   const size_t rows = std::min(l.size(), maxRows);
   if (features) std::copy(f.begin(), f.begin() + (ptrdiff_t)(rows * kMlpIn), features);
   if (labels) std::copy(l.begin(), l.begin() + (ptrdiff_t)rows, labels);
@@ -1154,6 +1186,7 @@ double renderPoseBatchAdaptive(const float* poses, size_t poseCount, float error
   upload_eye(*e);
   AdaptiveConfig cfg;
   cfg.error_target = errorTarget;
+  cfg.label_error = s.label_error;
   cfg.max_spp = maxSpp;
   cfg.budget_scale = budgetScale;
   AdaptiveResult res;
@@ -1261,6 +1294,31 @@ int64_t parseGlTFTriangles(const char* path, float* pos, float* col, uint32_t* m
     if (mode) memcpy(mode, hs.mode.data(), sizeof(uint32_t) * m);
   }
   return (int64_t)n;
+  ABI_GUARD_END(return -1)
+}
+
+int64_t parseGlTFTextures(const char* path, int32_t* desc, size_t maxTextures, uint32_t* texels, size_t maxTexels,
+                          int64_t* texelCount) {
+  ABI_GUARD_BEGIN
+  if (!path) throw std::runtime_error("parseGlTFTextures: null path");
+  HostScene hs;
+  std::string err;
+  if (!load_gltf(path, &hs, &err)) throw std::runtime_error(err);
+  const size_t nt = std::min(hs.textures.size(), maxTextures);
+  if (desc)
+    for (size_t t = 0; t < nt; ++t) {
+      const TextureDesc& d = hs.textures[t];
+      int32_t* o = desc + 6 * t;
+      o[0] = d.width;
+      o[1] = d.height;
+      o[2] = d.wrap_s;
+      o[3] = d.wrap_t;
+      o[4] = d.filter;
+      o[5] = (int32_t)d.first;
+    }
+  if (texels) memcpy(texels, hs.texels.data(), sizeof(uint32_t) * std::min(hs.texels.size(), maxTexels));
+  if (texelCount) *texelCount = (int64_t)hs.texels.size();
+  return (int64_t)hs.textures.size();
   ABI_GUARD_END(return -1)
 }
 

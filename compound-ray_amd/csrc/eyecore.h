@@ -276,14 +276,94 @@ EYE_HD float slab_enter(const RayPre& r, float lox, float hix, float loy, float 
 // ---------------------------------------------------------------------------
 constexpr float kGammaLinearize = 2.2f;
 
-enum ColourMode : uint32_t { kVertexColour = 0u, kBaseColour = 1u };
+enum ColourMode : uint32_t { kVertexColour = 0u, kBaseColour = 1u, kTextureColour = 2u };
+// The mode word of a triangle: ColourMode in bits 0-1, texture index above.
+EYE_HD uint32_t colour_mode(uint32_t word) { return word & 3u; }
+EYE_HD uint32_t texture_index(uint32_t word) { return word >> 2; }
+
+enum AddressMode : int { kAddressWrap = 0, kAddressClamp = 1, kAddressMirror = 2 };
+enum FilterMode : int { kFilterPoint = 0, kFilterLinear = 1 };
 
 EYE_HD V3 shade_hit(float b1, float b2, V3 c0, V3 c1, V3 c2, uint32_t mode) {
-  if (mode == kBaseColour) return c0;
+  if (colour_mode(mode) == kBaseColour) return c0;
   const float w0 = (1.0f - b1) - b2;
   const V3 c = (w0 * c0 + b1 * c1) + b2 * c2;
   return v3(det_pow(c.x, kGammaLinearize), det_pow(c.y, kGammaLinearize),
             det_pow(c.z, kGammaLinearize));
+}
+
+// A base-colour texture as the kernels see it: texels first .. first + w*h - 1
+// of the scene's RGBA8 texel array.
+struct TexView {
+  int width, height, wrap_s, wrap_t, filter;
+  uint32_t first;
+};
+
+// Texel index along one axis under an address mode: repeat, clamp to the
+// edge, or mirrored repeat (the three modes the reference maps glTF samplers
+// to, MulticamScene.cpp:613-621).
+EYE_HD int texel_index(int i, int n, int mode) {
+  if (mode == kAddressClamp) return i < 0 ? 0 : (i >= n ? n - 1 : i);
+  if (mode == kAddressMirror) {
+    const int period = 2 * n;
+    int m = i % period;
+    if (m < 0) m += period;
+    return m < n ? m : period - 1 - m;
+  }
+  int m = i % n;
+  if (m < 0) m += n;
+  return m;
+}
+
+// Normalised coordinate folded into one period first, so that texel indices
+// stay small: [0, 1) for repeat, [0, 2) for mirrored repeat, [0, 1] for clamp.
+EYE_HD float fold_coord(float u, int mode) {
+  if (!(u == u)) return 0.0f;
+  if (mode == kAddressClamp) return fminf(fmaxf(u, 0.0f), 1.0f);
+  if (mode == kAddressMirror) return u - 2.0f * floorf(u * 0.5f);
+  return u - floorf(u);
+}
+
+EYE_HD V3 texel_rgb(const uint32_t* texels, const TexView& t, int i, int j) {
+  const uint32_t v = texels[t.first + (uint32_t)j * (uint32_t)t.width + (uint32_t)i];
+  return v3((float)(v & 0xffu) / 255.0f, (float)((v >> 8) & 0xffu) / 255.0f, (float)((v >> 16) & 0xffu) / 255.0f);
+}
+
+// tex2D with normalised coordinates and normalised-float reads: point
+// sampling takes texel floor(u W); linear filtering blends the four texels
+// around (u W - 0.5, v H - 0.5) with weights quantised to 1/256 (the 8-bit
+// fraction of the texture unit's filter weights, as CUDA documents it).
+EYE_HD V3 sample_texture(const uint32_t* texels, const TexView& t, float u, float v) {
+  const float x = fold_coord(u, t.wrap_s) * (float)t.width;
+  const float y = fold_coord(v, t.wrap_t) * (float)t.height;
+  if (t.filter == kFilterPoint) {
+    const int i = texel_index((int)floorf(x), t.width, t.wrap_s);
+    const int j = texel_index((int)floorf(y), t.height, t.wrap_t);
+    return texel_rgb(texels, t, i, j);
+  }
+  const float xb = x - 0.5f, yb = y - 0.5f;
+  const float fx = floorf(xb), fy = floorf(yb);
+  const float a = rintf((xb - fx) * 256.0f) * (1.0f / 256.0f);
+  const float b = rintf((yb - fy) * 256.0f) * (1.0f / 256.0f);
+  const int i0 = texel_index((int)fx, t.width, t.wrap_s), i1 = texel_index((int)fx + 1, t.width, t.wrap_s);
+  const int j0 = texel_index((int)fy, t.height, t.wrap_t), j1 = texel_index((int)fy + 1, t.height, t.wrap_t);
+  const V3 t00 = texel_rgb(texels, t, i0, j0), t10 = texel_rgb(texels, t, i1, j0);
+  const V3 t01 = texel_rgb(texels, t, i0, j1), t11 = texel_rgb(texels, t, i1, j1);
+  const V3 top = t00 * (1.0f - a) + t10 * a;
+  const V3 bottom = t01 * (1.0f - a) + t11 * a;
+  return top * (1.0f - b) + bottom * b;
+}
+
+// Textured hit: texture coordinates interpolated like vertex colours
+// (LocalGeometry.h:92-103; without TEXCOORD_0 the loader stores (0,0),
+// (1,0), (0,1), which gives the barycentrics), then linearize of the texel
+// colour (shaders.cu:801-805).  uv holds u0 v0 u1 v1 u2 v2.
+EYE_HD V3 shade_texture(float b1, float b2, const float uv[6], const uint32_t* texels, const TexView& t) {
+  const float w0 = (1.0f - b1) - b2;
+  const float u = (w0 * uv[0] + b1 * uv[2]) + b2 * uv[4];
+  const float v = (w0 * uv[1] + b1 * uv[3]) + b2 * uv[5];
+  const V3 c = sample_texture(texels, t, u, v);
+  return v3(det_pow(c.x, kGammaLinearize), det_pow(c.y, kGammaLinearize), det_pow(c.z, kGammaLinearize));
 }
 
 // __miss__simple_sky, libEyeRenderer3/shaders.cu:749-756.

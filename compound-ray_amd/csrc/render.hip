@@ -18,6 +18,8 @@ struct SceneView {
   const WideNode* nodes;
   const float4* tris;
   const float4* cols;
+  const uint32_t* texels;
+  const TexView* textures;
   int node_count;
   int sky_mode;
 };
@@ -27,6 +29,8 @@ SceneView view_of(const DeviceScene& s) {
   v.nodes = s.nodes;
   v.tris = s.tris;
   v.cols = s.cols;
+  v.texels = s.texels;
+  v.textures = s.textures;
   v.node_count = s.node_count;
   v.sky_mode = s.sky_mode;
   return v;
@@ -170,9 +174,13 @@ __device__ __forceinline__ V3 shade(const SceneView& S, const Hit& h, float dx, 
   if (h.id == 0xffffffffu) return sky_colour(S.sky_mode, dx, dy, dz);
   const float4 c0 = S.cols[3 * (size_t)h.k + 0];
   const float4 c1 = S.cols[3 * (size_t)h.k + 1];
+  const uint32_t word = __float_as_uint(c0.w);
+  if (colour_mode(word) == kTextureColour) {
+    const float uv[6] = {c0.x, c0.y, c0.z, c1.x, c1.y, c1.z};
+    return shade_texture(h.b1, h.b2, uv, S.texels, S.textures[texture_index(word)]);
+  }
   const float4 c2 = S.cols[3 * (size_t)h.k + 2];
-  return shade_hit(h.b1, h.b2, v3(c0.x, c0.y, c0.z), v3(c1.x, c1.y, c1.z), v3(c2.x, c2.y, c2.z),
-                   __float_as_uint(c0.w));
+  return shade_hit(h.b1, h.b2, v3(c0.x, c0.y, c0.z), v3(c1.x, c1.y, c1.z), v3(c2.x, c2.y, c2.z), word);
 }
 
 // Maps a ray number to (pose, ommatidium, sample, samples of that ommatidium).

@@ -8,6 +8,7 @@
 #include <sstream>
 
 #include "json.h"
+#include "png.h"
 
 namespace eye {
 namespace {
@@ -147,6 +148,9 @@ struct Loader {
   std::vector<std::vector<uint8_t>> buffers;
   HostScene* out = nullptr;
   std::string err;
+  std::vector<int> texture_slot;   // glTF texture -> HostScene::textures index (-1: unusable, -2: not yet seen)
+  std::vector<int> image_first;    // glTF image -> first texel (-1: not decoded yet)
+  std::vector<int> image_w, image_h;
 
   struct Accessor {
     const uint8_t* base = nullptr;
@@ -251,6 +255,117 @@ struct Loader {
 
   // Vertex colour decode of cuda/LocalGeometry.h:106-146 (u8 / 255, u16 / 65535, f32 as is).
   static float get_colour(const Accessor& a, size_t i, int c) {
+    const uint8_t* p = a.base + a.stride * i + (size_t)component_bytes(a.ctype) * c;
+    switch (a.ctype) {
+      case 5121: return (float)(*p) / 255.0f;
+      case 5123: { uint16_t v; memcpy(&v, p, 2); return (float)v / 65535.0f; }
+      default: { float f; memcpy(&f, p, 4); return f; }
+    }
+  }
+
+  // Bytes of glTF image `idx`: a bufferView, a base64 data URI or a file
+  // next to the glTF.
+  bool image_bytes(int idx, std::vector<uint8_t>* bytes) {
+    const JsonValue& im = doc["images"][(size_t)idx];
+    const int bv_idx = im["bufferView"].as_int(-1);
+    if (bv_idx >= 0) {
+      const JsonValue& bv = doc["bufferViews"][(size_t)bv_idx];
+      const int buf = bv["buffer"].as_int(-1);
+      const size_t off = (size_t)bv["byteOffset"].as_number(0.0);
+      const size_t len = (size_t)bv["byteLength"].as_number(0.0);
+      if (buf < 0 || (size_t)buf >= buffers.size() || off + len > buffers[(size_t)buf].size()) {
+        err = "image " + std::to_string(idx) + ": bad bufferView";
+        return false;
+      }
+      bytes->assign(buffers[(size_t)buf].begin() + (ptrdiff_t)off, buffers[(size_t)buf].begin() + (ptrdiff_t)(off + len));
+      return true;
+    }
+    const std::string uri = im["uri"].as_string();
+    if (uri.rfind("data:", 0) == 0) {
+      const size_t comma = uri.find(',');
+      if (comma == std::string::npos || !b64_decode(uri.data() + comma + 1, uri.size() - comma - 1, bytes)) {
+        err = "image " + std::to_string(idx) + ": unsupported data URI";
+        return false;
+      }
+      return true;
+    }
+    std::string file;
+    if (uri.empty() || (!read_file(dir + uri, &file) && !read_file(uri, &file))) {
+      err = "image " + std::to_string(idx) + ": cannot open '" + uri + "'";
+      return false;
+    }
+    bytes->assign(file.begin(), file.end());
+    return true;
+  }
+
+  // HostScene texture index for glTF texture `t`, decoding its image on first
+  // use; -1 (with a warning) when it cannot be sampled.  A texture without a
+  // sampler repeats and filters linearly (MulticamScene.cpp:606-610); with
+  // one, minFilter NEAREST selects point sampling (:619-621).
+  int texture(int t) {
+    const JsonValue& texs = doc["textures"];
+    if (t < 0 || (size_t)t >= texs.size()) return -1;
+    if (texture_slot.empty()) texture_slot.assign(texs.size(), -2);
+    if (texture_slot[(size_t)t] != -2) return texture_slot[(size_t)t];
+    texture_slot[(size_t)t] = -1;
+    const JsonValue& tx = texs[(size_t)t];
+    const int src = tx["source"].as_int(-1);
+    const JsonValue& imgs = doc["images"];
+    if (src < 0 || (size_t)src >= imgs.size()) {
+      out->warnings.push_back("texture " + std::to_string(t) + ": no image");
+      return -1;
+    }
+    if (image_first.empty()) {
+      image_first.assign(imgs.size(), -1);
+      image_w.assign(imgs.size(), 0);
+      image_h.assign(imgs.size(), 0);
+    }
+    if (image_first[(size_t)src] < 0) {
+      std::vector<uint8_t> bytes, rgba;
+      std::string perr;
+      int w = 0, h = 0;
+      if (!image_bytes(src, &bytes)) {
+        out->warnings.push_back(err + "; texture " + std::to_string(t) + " not sampled");
+        err.clear();
+        return -1;
+      }
+      if (!png_decode(bytes.data(), bytes.size(), &w, &h, &rgba, &perr)) {
+        out->warnings.push_back("image " + std::to_string(src) + ": " + perr +
+                                " (only PNG images are decoded); texture " + std::to_string(t) + " not sampled");
+        return -1;
+      }
+      if (out->texels.size() + (size_t)w * (size_t)h > 0x7fffffffu) {
+        out->warnings.push_back("image " + std::to_string(src) + ": texel budget exceeded, not sampled");
+        return -1;
+      }
+      image_first[(size_t)src] = (int)out->texels.size();
+      image_w[(size_t)src] = w;
+      image_h[(size_t)src] = h;
+      for (size_t k = 0; k < (size_t)w * (size_t)h; ++k)
+        out->texels.push_back((uint32_t)rgba[4 * k] | (uint32_t)rgba[4 * k + 1] << 8 |
+                              (uint32_t)rgba[4 * k + 2] << 16 | (uint32_t)rgba[4 * k + 3] << 24);
+    }
+    TextureDesc d;
+    d.first = (uint32_t)image_first[(size_t)src];
+    d.width = image_w[(size_t)src];
+    d.height = image_h[(size_t)src];
+    const int si = tx["sampler"].as_int(-1);
+    if (si >= 0) {
+      const JsonValue& sm = doc["samplers"][(size_t)si];
+      auto mode_of = [](int gl) {
+        return gl == 33071 ? kAddressClamp : gl == 33648 ? kAddressMirror : kAddressWrap;
+      };
+      d.wrap_s = mode_of(sm["wrapS"].as_int(10497));
+      d.wrap_t = mode_of(sm["wrapT"].as_int(10497));
+      d.filter = sm["minFilter"].as_int(-1) == 9728 ? kFilterPoint : kFilterLinear;
+    }
+    texture_slot[(size_t)t] = (int)out->textures.size();
+    out->textures.push_back(d);
+    return texture_slot[(size_t)t];
+  }
+
+  // Texture coordinate of a vertex: float, or normalised unsigned byte/short.
+  static float get_texcoord(const Accessor& a, size_t i, int c) {
     const uint8_t* p = a.base + a.stride * i + (size_t)component_bytes(a.ctype) * c;
     switch (a.ctype) {
       case 5121: return (float)(*p) / 255.0f;
@@ -410,7 +525,10 @@ struct Loader {
           out->warnings.push_back("mesh '" + name + "': COLOR_0 is not VEC4 f32/u16/u8, vertex colours ignored");
         }
       }
+      // Colour source in the reference's order (shaders.cu:788-806): vertex
+      // colours, else the base-colour texture, else the base colour factor.
       float base[3] = {1.0f, 1.0f, 1.0f};
+      int tex = -1;
       const int mi = prim["material"].as_int(-1);
       if (mi >= 0) {
         const JsonValue& mat = doc["materials"][(size_t)mi];
@@ -418,9 +536,27 @@ struct Loader {
         const JsonValue& bcf = pbr["baseColorFactor"];
         if (bcf.size() >= 3)
           for (int a = 0; a < 3; ++a) base[a] = (float)bcf[(size_t)a].num;
-        if (!use_vc && pbr["baseColorTexture"].is_object())
-          out->warnings.push_back("mesh '" + name + "': base colour textures are not sampled; using baseColorFactor");
+        if (!use_vc && pbr["baseColorTexture"].is_object()) {
+          tex = texture(pbr["baseColorTexture"]["index"].as_int(-1));
+          if (tex < 0) out->warnings.push_back("mesh '" + name + "': base colour texture unusable; using baseColorFactor");
+        }
       }
+      Accessor ta;
+      bool has_uv = false;
+      if (tex >= 0) {
+        const int ti = attrs["TEXCOORD_0"].as_int(-1);
+        if (ti >= 0) {
+          if (!accessor(ti, &ta)) return false;
+          if (ta.ncomp != 2 || (ta.ctype != 5126 && ta.ctype != 5121 && ta.ctype != 5123) || ta.count < pa.count) {
+            err = "mesh '" + name + "': TEXCOORD_0 must be VEC2 float / normalised u8 / u16 covering POSITION";
+            return false;
+          }
+          has_uv = true;
+        }
+      }
+      const uint32_t mode_word = use_vc ? (uint32_t)kVertexColour
+                                 : tex >= 0 ? ((uint32_t)kTextureColour | (uint32_t)tex << 2)
+                                            : (uint32_t)kBaseColour;
       const size_t ntri = ii >= 0 ? ia.count / 3 : pa.count / 3;
       out->mesh_names.push_back(name);
       out->mesh_first_tri.push_back((uint32_t)out->mode.size());
@@ -449,13 +585,24 @@ struct Loader {
             out->col.push_back(get_colour(ca, vi, 0));
             out->col.push_back(get_colour(ca, vi, 1));
             out->col.push_back(get_colour(ca, vi, 2));
+          } else if (tex >= 0) {
+            // two texture coordinates per vertex, then 3 unused slots
+            if (has_uv) {
+              out->col.push_back(get_texcoord(ta, vi, 0));
+              out->col.push_back(get_texcoord(ta, vi, 1));
+            } else {
+              out->col.push_back(v == 1 ? 1.0f : 0.0f);
+              out->col.push_back(v == 2 ? 1.0f : 0.0f);
+            }
           } else {
             out->col.push_back(base[0]);
             out->col.push_back(base[1]);
             out->col.push_back(base[2]);
           }
         }
-        out->mode.push_back(use_vc ? kVertexColour : kBaseColour);
+        if (tex >= 0 && !use_vc)
+          for (int z = 0; z < 3; ++z) out->col.push_back(0.0f);
+        out->mode.push_back(mode_word);
       }
       out->mesh_bounds.push_back(mb);
     }

@@ -33,10 +33,24 @@ struct HitboxMesh {
   float lo[3], hi[3];              // world AABB
 };
 
+// Base-colour texture: an RGBA8 image (texels first .. first + w*h - 1 of
+// HostScene::texels, row 0 = top row) with its sampler's address modes and
+// filter (MulticamScene.cpp:606-623, 801-834).
+struct TextureDesc {
+  int width = 0, height = 0;
+  int wrap_s = kAddressWrap, wrap_t = kAddressWrap;
+  int filter = kFilterLinear;
+  uint32_t first = 0;
+};
+
 struct HostScene {
   std::vector<float> pos;          // 9 floats per triangle, world space
-  std::vector<float> col;          // 9 floats per triangle (vertex colours or base colour)
-  std::vector<uint32_t> mode;      // ColourMode per triangle
+  // 9 floats per triangle: vertex colours, the base colour (x3), or for a
+  // textured triangle its texture coordinates (u0 v0 u1 v1 u2 v2 0 0 0)
+  std::vector<float> col;
+  std::vector<uint32_t> mode;      // ColourMode per triangle (| texture index << 2)
+  std::vector<uint32_t> texels;    // RGBA8, red in the low byte
+  std::vector<TextureDesc> textures;
   std::vector<std::string> mesh_names;
   std::vector<uint32_t> mesh_first_tri;
   std::vector<std::array<float, 6>> mesh_bounds;  // world AABB per mesh primitive group (lo, hi)

@@ -132,6 +132,8 @@ def configure_extensions(lib):
     lib.renderPoseBatchAdaptive.argtypes = [c_void_p, c_size_t, c_float, c_int, c_float, c_void_p, c_void_p,
                                             c_void_p]
     lib.renderPoseBatchAdaptive.restype = c_double
+    lib.setAdaptiveLabelError.argtypes = [c_float]
+    lib.setAdaptiveLabelError.restype = c_int
     lib.setTraversalMode.argtypes = [c_int]
     lib.setSampleRecording.argtypes = [c_bool]
     lib.setRenderTuning.argtypes = [c_int, c_int]
@@ -161,6 +163,8 @@ def configure_extensions(lib):
     lib.setSceneLeafSize.restype = c_int
     lib.parseGlTFTriangles.argtypes = [c_char_p, c_void_p, c_void_p, c_void_p, c_size_t]
     lib.parseGlTFTriangles.restype = c_int64
+    lib.parseGlTFTextures.argtypes = [c_char_p, c_void_p, c_size_t, c_void_p, c_size_t, c_void_p]
+    lib.parseGlTFTextures.restype = c_int64
 
 
 # ---------------------------------------------------------------- helpers ----
@@ -411,3 +415,16 @@ def parse_gltf_np(lib, path: str):
     lib.parseGlTFTriangles(path.encode(), pos.ctypes.data_as(c_void_p), col.ctypes.data_as(c_void_p),
                            mode.ctypes.data_as(c_void_p), n)
     return pos, col, mode
+
+
+def parse_textures_np(lib, path: str):
+    """(descriptors (n, 6) int32, texels uint32) of the glTF's base-colour textures."""
+    count = np.zeros(1, np.int64)
+    n = lib.parseGlTFTextures(path.encode(), None, 0, None, 0, count.ctypes.data_as(c_void_p))
+    if n < 0:
+        raise RuntimeError(last_error(lib))
+    desc = np.zeros((n, 6), np.int32)
+    texels = np.zeros(int(count[0]), np.uint32)
+    lib.parseGlTFTextures(path.encode(), desc.ctypes.data_as(c_void_p), n, texels.ctypes.data_as(c_void_p),
+                          texels.size, count.ctypes.data_as(c_void_p))
+    return desc, texels

@@ -30,25 +30,98 @@ import numpy as np
 DEG = math.pi / 180.0
 
 
+GL_REPEAT, GL_CLAMP_TO_EDGE, GL_MIRRORED_REPEAT = 10497, 33071, 33648
+GL_NEAREST, GL_LINEAR = 9728, 9729
+_ADDRESS = {GL_REPEAT: 0, GL_CLAMP_TO_EDGE: 1, GL_MIRRORED_REPEAT: 2}
+
+
+@dataclass
+class Texture:
+    """RGBA8 image (row 0 = top) with a glTF sampler."""
+    rgba: np.ndarray                 # (H, W, 4) uint8
+    wrap_s: int = GL_REPEAT
+    wrap_t: int = GL_REPEAT
+    min_filter: int | None = GL_LINEAR   # None: texture written without a sampler
+    embed: bool = True               # PNG in the .bin buffer, else a file next to the .gltf
+
+    def desc(self) -> tuple:
+        """(wrap s, wrap t, filter) as the renderer encodes them."""
+        if self.min_filter is None:
+            return 0, 0, 1
+        return _ADDRESS[self.wrap_s], _ADDRESS[self.wrap_t], 0 if self.min_filter == GL_NEAREST else 1
+
+
 @dataclass
 class Mesh:
     name: str
     pos: np.ndarray          # (V, 3) float32
     idx: np.ndarray          # (T, 3) uint32
-    col: np.ndarray | None   # (V, 4) float32, or None (material colour)
+    col: np.ndarray | None   # (V, 4) float32, or None (material colour / texture)
     base_colour: tuple = (1.0, 1.0, 1.0, 1.0)
+    texture: Texture | None = None   # base-colour texture (used when col is None)
+    uv: np.ndarray | None = None     # (V, 2) float32 TEXCOORD_0; None: barycentric UVs
+
+    def textured(self) -> bool:
+        return self.col is None and self.texture is not None
 
     def tri_pos(self) -> np.ndarray:
         return self.pos[self.idx].reshape(-1, 9).astype(np.float32)
 
     def tri_col(self) -> np.ndarray:
+        if self.textured():
+            t = self.idx.shape[0]
+            out = np.zeros((t, 9), np.float32)
+            if self.uv is None:
+                out[:, 0:6] = np.array([0, 0, 1, 0, 0, 1], np.float32)
+            else:
+                out[:, 0:6] = self.uv.astype(np.float32)[self.idx].reshape(-1, 6)
+            return out
         if self.col is None:
             c = np.asarray(self.base_colour[:3], dtype=np.float32)
             return np.tile(np.concatenate([c, c, c]), (self.idx.shape[0], 1))
         return self.col[self.idx][:, :, :3].reshape(-1, 9).astype(np.float32)
 
-    def tri_mode(self) -> np.ndarray:
+    def tri_mode(self, texture_index: int = 0) -> np.ndarray:
+        if self.textured():
+            return np.full(self.idx.shape[0], 2 | (texture_index << 2), dtype=np.uint32)
         return np.full(self.idx.shape[0], 0 if self.col is not None else 1, dtype=np.uint32)
+
+
+def encode_png(rgba: np.ndarray, filters=(0, 1, 2, 3, 4)) -> bytes:
+    """RGBA8 PNG; row filter types cycle through `filters`."""
+    import struct
+    import zlib
+    img = np.ascontiguousarray(rgba, dtype=np.uint8)
+    h, w = img.shape[:2]
+    rows = img.reshape(h, w * 4).astype(np.int32)
+    raw = bytearray()
+    for y in range(h):
+        ft = filters[y % len(filters)]
+        cur = rows[y]
+        prev = rows[y - 1] if y > 0 else np.zeros_like(cur)
+        left = np.concatenate([np.zeros(4, np.int32), cur[:-4]])
+        upleft = np.concatenate([np.zeros(4, np.int32), prev[:-4]])
+        if ft == 0:
+            pred = np.zeros_like(cur)
+        elif ft == 1:
+            pred = left
+        elif ft == 2:
+            pred = prev
+        elif ft == 3:
+            pred = (left + prev) >> 1
+        else:
+            p = left + prev - upleft
+            pa, pb, pc = np.abs(p - left), np.abs(p - prev), np.abs(p - upleft)
+            pred = np.where((pa <= pb) & (pa <= pc), left, np.where(pb <= pc, prev, upleft))
+        raw.append(ft)
+        raw.extend(((cur - pred) & 255).astype(np.uint8).tobytes())
+
+    def chunk(kind: bytes, body: bytes) -> bytes:
+        return struct.pack(">I", len(body)) + kind + body + struct.pack(">I", zlib.crc32(kind + body) & 0xffffffff)
+
+    ihdr = struct.pack(">IIBBBBB", w, h, 8, 6, 0, 0, 0)
+    return b"\x89PNG\r\n\x1a\n" + chunk(b"IHDR", ihdr) + chunk(b"IDAT", zlib.compress(bytes(raw), 6)) + \
+        chunk(b"IEND", b"")
 
 
 @dataclass
@@ -73,8 +146,37 @@ class Scene:
     def tri_col(self) -> np.ndarray:
         return np.concatenate([m.tri_col() for m in self.meshes]) if self.meshes else np.zeros((0, 9), np.float32)
 
+    def textures(self) -> list:
+        """Distinct textures of textured meshes, in first-use order."""
+        out = []
+        for m in self.meshes:
+            if m.textured() and not any(t is m.texture for t in out):
+                out.append(m.texture)
+        return out
+
     def tri_mode(self) -> np.ndarray:
-        return np.concatenate([m.tri_mode() for m in self.meshes]) if self.meshes else np.zeros(0, np.uint32)
+        if not self.meshes:
+            return np.zeros(0, np.uint32)
+        texs = self.textures()
+        parts = []
+        for m in self.meshes:
+            ti = next((k for k, t in enumerate(texs) if t is m.texture), 0) if m.textured() else 0
+            parts.append(m.tri_mode(ti))
+        return np.concatenate(parts)
+
+    def texture_arrays(self):
+        """(descriptors (n, 6) int32: width, height, wrap s, wrap t, filter,
+        first texel; texels uint32 RGBA8 with red in the low byte)."""
+        desc, texels, first = [], [], 0
+        for t in self.textures():
+            h, w = t.rgba.shape[:2]
+            desc.append((w, h) + t.desc() + (first,))
+            px = t.rgba.astype(np.uint32).reshape(-1, 4)
+            texels.append(px[:, 0] | px[:, 1] << 8 | px[:, 2] << 16 | px[:, 3] << 24)
+            first += w * h
+        if not desc:
+            return np.zeros((0, 6), np.int32), np.zeros(0, np.uint32)
+        return np.array(desc, np.int32), np.concatenate(texels).astype(np.uint32)
 
     def tri_count(self) -> int:
         return int(sum(m.idx.shape[0] for m in self.meshes))
@@ -246,6 +348,33 @@ def write_gltf(directory, name: str, scene: Scene) -> pathlib.Path:
         views.append({"buffer": 0, "byteOffset": off, "byteLength": len(data), "target": target})
         return len(views) - 1
 
+    images, samplers, textures = [], [], []
+    tex_index = {}
+
+    def add_texture(t: Texture) -> int:
+        if id(t) in tex_index:
+            return tex_index[id(t)]
+        png = encode_png(t.rgba)
+        if t.embed:
+            while len(blob) % 4:
+                blob.append(0)
+            off = len(blob)
+            blob.extend(png)
+            views.append({"buffer": 0, "byteOffset": off, "byteLength": len(png)})
+            images.append({"bufferView": len(views) - 1, "mimeType": "image/png"})
+        else:
+            fname = f"{name}-tex{len(images)}.png"
+            (d / fname).write_bytes(png)
+            images.append({"uri": fname})
+        tex = {"source": len(images) - 1}
+        if t.min_filter is not None:
+            samplers.append({"wrapS": t.wrap_s, "wrapT": t.wrap_t, "minFilter": t.min_filter,
+                             "magFilter": GL_LINEAR})
+            tex["sampler"] = len(samplers) - 1
+        textures.append(tex)
+        tex_index[id(t)] = len(textures) - 1
+        return tex_index[id(t)]
+
     for m in scene.meshes:
         pv = add_view(m.pos.astype(np.float32), 34962)
         accessors.append({"bufferView": pv, "componentType": 5126, "count": int(m.pos.shape[0]), "type": "VEC3",
@@ -255,10 +384,16 @@ def write_gltf(directory, name: str, scene: Scene) -> pathlib.Path:
             cv = add_view(m.col.astype(np.float32), 34962)
             accessors.append({"bufferView": cv, "componentType": 5126, "count": int(m.col.shape[0]), "type": "VEC4"})
             attrs["COLOR_0"] = len(accessors) - 1
+        if m.uv is not None:
+            uvv = add_view(m.uv.astype(np.float32), 34962)
+            accessors.append({"bufferView": uvv, "componentType": 5126, "count": int(m.uv.shape[0]), "type": "VEC2"})
+            attrs["TEXCOORD_0"] = len(accessors) - 1
         iv = add_view(m.idx.astype(np.uint32).reshape(-1), 34963)
         accessors.append({"bufferView": iv, "componentType": 5125, "count": int(m.idx.size), "type": "SCALAR"})
-        materials.append({"name": m.name + "-material",
-                          "pbrMetallicRoughness": {"baseColorFactor": [float(v) for v in m.base_colour]}})
+        pbr = {"baseColorFactor": [float(v) for v in m.base_colour]}
+        if m.texture is not None:
+            pbr["baseColorTexture"] = {"index": add_texture(m.texture)}
+        materials.append({"name": m.name + "-material", "pbrMetallicRoughness": pbr})
         meshes.append({"name": m.name, "primitives": [{"attributes": attrs, "indices": len(accessors) - 1,
                                                        "material": len(materials) - 1, "mode": 4}]})
         nodes.append({"name": m.name, "mesh": len(meshes) - 1})
@@ -281,6 +416,10 @@ def write_gltf(directory, name: str, scene: Scene) -> pathlib.Path:
         "accessors": accessors, "bufferViews": views,
         "buffers": [{"uri": bin_name, "byteLength": len(blob)}],
     }
+    if textures:
+        doc.update({"images": images, "textures": textures})
+        if samplers:
+            doc["samplers"] = samplers
     path = d / f"{name}.gltf"
     path.write_text(json.dumps(doc))
     return path
@@ -310,6 +449,35 @@ def config2_scene(seed: int = 2, grid: int = 1024, extent: float = 40.0, n_cyl: 
     terrain, hf = heightfield(grid, extent, 1.5, 6, 0.08, rng, colour="earth")
     veg = cylinders(n_cyl, hf, rng, extent)
     return Scene([terrain, veg], heightfield=hf)
+
+
+def textured_scene(seed: int = 11) -> Scene:
+    """Textured test scene over 10 m: a heightfield whose repeating base-colour
+    texture (linear filter) is mapped by world x/z, cylinders with a second
+    texture (point filter, mirrored/clamped addressing, barycentric UVs), and
+    vertex-coloured cylinders."""
+    rng = np.random.default_rng(seed)
+    terrain, hf = heightfield(60, 10.0, 0.4, 3, 0.3, rng)
+    gy, gx = np.mgrid[0:48, 0:64]
+    img = np.zeros((48, 64, 4), np.uint8)
+    img[..., 0] = (128 + 100 * np.sin(gx * 0.35) * np.cos(gy * 0.2)).astype(np.uint8)
+    img[..., 1] = rng.integers(40, 220, size=(48, 64))
+    img[..., 2] = ((gx * 4 + gy * 3) % 256).astype(np.uint8)
+    img[..., 3] = 255
+    terrain.col = None
+    terrain.texture = Texture(img, GL_REPEAT, GL_REPEAT, GL_LINEAR)
+    terrain.uv = np.stack([terrain.pos[:, 0] * 0.3, terrain.pos[:, 2] * 0.3], axis=1).astype(np.float32)
+    stripes = np.zeros((8, 5, 4), np.uint8)
+    stripes[..., 0] = (np.arange(5) * 60)[None, :]
+    stripes[..., 1] = (np.arange(8) * 30)[:, None]
+    stripes[..., 2] = 200
+    stripes[..., 3] = 255
+    veg = cylinders(60, hf, rng, 10.0)
+    veg.col = None
+    veg.texture = Texture(stripes, GL_MIRRORED_REPEAT, GL_CLAMP_TO_EDGE, GL_NEAREST, embed=False)
+    veg2 = cylinders(40, hf, rng, 10.0)
+    veg2.name = "vegetation-coloured"
+    return Scene([terrain, veg, veg2], heightfield=hf)
 
 
 def config2_eyes(seed: int = 2, n: int = 5500):

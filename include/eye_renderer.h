@@ -184,8 +184,13 @@ int64_t adaptiveTrainingSet(const float* poses, size_t poseCount, const uint8_t*
                             float errorTarget, float* features, float* labels, size_t maxRows);
 /* Batched adaptive render: 8-sample pilot, MLP prediction, per-pose Neyman
  * allocation for RMS error errorTarget (times budgetScale), top-up, combine.
+ * The network predicts the samples needed for the label error (the
+ * errorTarget of the last adaptiveTrainingSet call, or setAdaptiveLabelError);
+ * allocations scale by (label error / errorTarget)^2.
  * rgbOut / countsOut (nullable) per (pose, ommatidium); *totalSamples gets the
  * samples traced.  Returns GPU milliseconds, or -1. */
+/* Error the sampler network's labels were made for (default 0.01). */
+int setAdaptiveLabelError(float labelError);
 double renderPoseBatchAdaptive(const float* poses, size_t poseCount, float errorTarget, int maxSpp,
                                float budgetScale, float* rgbOut, uint32_t* countsOut, uint64_t* totalSamples);
 
@@ -234,6 +239,14 @@ double runCopyProbe(size_t bytes, int repeats);
  * (9 floats) and colour modes of `path`.  Returns the triangle count (call
  * with maxTris = 0 to query), or -1 on error. */
 int64_t parseGlTFTriangles(const char* path, float* pos, float* col, uint32_t* mode, size_t maxTris);
+/* Host-only parse of the base-colour textures of `path`: 6 ints per texture
+ * (width, height, wrap s, wrap t, filter, first texel; wrap 0 repeat, 1 clamp,
+ * 2 mirrored repeat; filter 0 point, 1 linear) and the decoded RGBA8 texels
+ * (red in the low byte).  A textured triangle's mode is 2 | texture << 2 and
+ * its colour slots hold u0 v0 u1 v1 u2 v2.  Returns the texture count and
+ * sets *texelCount, or -1 on error. */
+int64_t parseGlTFTextures(const char* path, int32_t* desc, size_t maxTextures, uint32_t* texels, size_t maxTexels,
+                          int64_t* texelCount);
 
 #if defined(__GNUC__)
 #pragma GCC visibility pop

@@ -40,6 +40,8 @@ def lib():
         L.oracle_scene_create.argtypes = [c_void_p, c_void_p, c_void_p, c_int64, c_int, c_int]
         L.oracle_scene_create.restype = c_void_p
         L.oracle_scene_free.argtypes = [c_void_p]
+        L.oracle_scene_set_textures.argtypes = [c_void_p, c_void_p, c_int64, c_void_p, c_int64]
+        L.oracle_texture_sample.argtypes = [c_void_p, c_uint32, c_float, c_float, c_void_p]
         L.oracle_scene_info.argtypes = [c_void_p, c_void_p]
         L.oracle_scene_nodes.argtypes = [c_void_p, c_void_p]
         L.oracle_scene_order.argtypes = [c_void_p, c_void_p, c_void_p]
@@ -99,16 +101,29 @@ def make_color(rgb) -> np.ndarray:
 class OracleScene:
     """A scene plus its CPU-built LBVH (same rules as the GPU builder)."""
 
-    def __init__(self, tri_pos, tri_col, tri_mode, sky_mode: int = 1, leaf_size: int = 4):
+    def __init__(self, tri_pos, tri_col, tri_mode, sky_mode: int = 1, leaf_size: int = 4, textures=None):
+        """textures: optional (descriptors (n, 6) int32, texels uint32 RGBA8)."""
         self.pos = np.ascontiguousarray(tri_pos, dtype=np.float32).reshape(-1, 9)
         self.col = np.ascontiguousarray(tri_col, dtype=np.float32).reshape(-1, 9)
         self.mode = np.ascontiguousarray(tri_mode, dtype=np.uint32).reshape(-1)
         self.n = self.pos.shape[0]
         self.h = lib().oracle_scene_create(_p(self.pos), _p(self.col), _p(self.mode), self.n, sky_mode, leaf_size)
+        if textures is not None and len(textures[0]):
+            self.tex_desc = np.ascontiguousarray(textures[0], dtype=np.int32).reshape(-1, 6)
+            self.texels = np.ascontiguousarray(textures[1], dtype=np.uint32).reshape(-1)
+            lib().oracle_scene_set_textures(self.h, _p(self.tex_desc), self.tex_desc.shape[0], _p(self.texels),
+                                            self.texels.size)
 
     @classmethod
     def from_scene(cls, scene, leaf_size: int = 4):
-        return cls(scene.tri_pos(), scene.tri_col(), scene.tri_mode(), scene.sky_mode(), leaf_size)
+        return cls(scene.tri_pos(), scene.tri_col(), scene.tri_mode(), scene.sky_mode(), leaf_size,
+                   textures=scene.texture_arrays())
+
+    def texture_sample(self, tex: int, u: float, v: float) -> np.ndarray:
+        """One lookup of texture `tex` at (u, v): linear RGB before linearize."""
+        out = np.zeros(3, np.float32)
+        lib().oracle_texture_sample(self.h, tex, u, v, _p(out))
+        return out
 
     def __del__(self):
         if getattr(self, "h", None) and _lib is not None:

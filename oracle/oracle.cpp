@@ -333,6 +333,9 @@ struct Scene {
   std::vector<float> pos;   // 9 per triangle, original order
   std::vector<float> col;   // 9 per triangle
   std::vector<uint32_t> mode;
+  // base-colour textures: (width, height, wrap s, wrap t, filter, first texel) each
+  std::vector<int32_t> tex;
+  std::vector<uint32_t> texels;      // RGBA8, red in the low byte
   std::vector<uint32_t> codes, ids;  // sorted
   std::vector<Node4> nodes;
   std::vector<float> stris;  // 9 per sorted triangle
@@ -620,10 +623,66 @@ F3 sky(int mode, const float d_in[3]) {
   return f3((o_atan2(d.z, d.x) + PI_F) / (PI_F * 2.0f), (o_asin(d.y) + PI_F / 2.0f) / PI_F, 0.0f);
 }
 
+// Texel coordinate along one axis: repeat wraps modulo n, clamp holds the
+// edge texel, mirrored repeat reflects every other period.
+int axis_texel(int i, int n, int addr) {
+  switch (addr) {
+    case 1: return std::min(std::max(i, 0), n - 1);
+    case 2: {
+      const int r = ((i % (2 * n)) + 2 * n) % (2 * n);
+      return r >= n ? 2 * n - 1 - r : r;
+    }
+    default: return ((i % n) + n) % n;
+  }
+}
+
+F3 texel(const Scene& S, const int32_t* T, int i, int j) {
+  const uint32_t v = S.texels[(size_t)T[5] + (size_t)j * (size_t)T[0] + (size_t)i];
+  return f3((float)(v & 255u) / 255.0f, (float)(v >> 8 & 255u) / 255.0f, (float)(v >> 16 & 255u) / 255.0f);
+}
+
+// tex2D(normalised coords, normalised float read) for the reference's
+// samplers (MulticamScene.cpp:801-834): the coordinate is first reduced to one
+// period of its address mode; point filtering takes texel floor(u*W); linear
+// filtering weights the 2x2 texels around (u*W - 0.5) by fractions rounded to
+// multiples of 1/256.
+F3 texture_rgb(const Scene& S, uint32_t t, float u, float v) {
+  const int32_t* T = &S.tex[6 * (size_t)t];
+  auto reduce = [](float c, int addr) -> float {
+    if (std::isnan(c)) return 0.0f;
+    if (addr == 1) return std::fmin(std::fmax(c, 0.0f), 1.0f);
+    if (addr == 2) return c - 2.0f * std::floor(c * 0.5f);
+    return c - std::floor(c);
+  };
+  const float x = reduce(u, T[2]) * (float)T[0];
+  const float y = reduce(v, T[3]) * (float)T[1];
+  if (T[4] == 0)
+    return texel(S, T, axis_texel((int)std::floor(x), T[0], T[2]), axis_texel((int)std::floor(y), T[1], T[3]));
+  const float xs = x - 0.5f, ys = y - 0.5f;
+  const float x0 = std::floor(xs), y0 = std::floor(ys);
+  const float wx = std::nearbyint((xs - x0) * 256.0f) * (1.0f / 256.0f);
+  const float wy = std::nearbyint((ys - y0) * 256.0f) * (1.0f / 256.0f);
+  const int ix = (int)x0, iy = (int)y0;
+  const int xa = axis_texel(ix, T[0], T[2]), xb = axis_texel(ix + 1, T[0], T[2]);
+  const int ya = axis_texel(iy, T[1], T[3]), yb = axis_texel(iy + 1, T[1], T[3]);
+  const F3 r0 = add(scale(texel(S, T, xa, ya), 1.0f - wx), scale(texel(S, T, xb, ya), wx));
+  const F3 r1 = add(scale(texel(S, T, xa, yb), 1.0f - wx), scale(texel(S, T, xb, yb), wx));
+  return add(scale(r0, 1.0f - wy), scale(r1, wy));
+}
+
 F3 shade(const Scene& S, const HitO& h, const RayO& r) {
   if (h.id == 0xffffffffu) return sky(S.sky, r.d);
   const float* c = &S.col[9 * (size_t)h.id];
-  if (S.mode[h.id] == 1u) return f3(c[0], c[1], c[2]);  // material base colour, no linearize
+  const uint32_t m = S.mode[h.id];
+  if ((m & 3u) == 1u) return f3(c[0], c[1], c[2]);  // material base colour, no linearize
+  if ((m & 3u) == 2u) {
+    // textured: UV interpolated as in LocalGeometry.h:92-103, then linearize (shaders.cu:801-805)
+    const float w0 = (1.0f - h.b1) - h.b2;
+    const float u = (w0 * c[0] + h.b1 * c[2]) + h.b2 * c[4];
+    const float v = (w0 * c[1] + h.b1 * c[3]) + h.b2 * c[5];
+    const F3 t = texture_rgb(S, m >> 2, u, v);
+    return f3(o_pow(t.x, 2.2f), o_pow(t.y, 2.2f), o_pow(t.z, 2.2f));
+  }
   const float w0 = (1.0f - h.b1) - h.b2;                 // LocalGeometry.h:148
   F3 v;
   v.x = (w0 * c[0] + h.b1 * c[3]) + h.b2 * c[6];
@@ -696,6 +755,22 @@ void* oracle_scene_create(const float* pos, const float* col, const uint32_t* mo
 }
 
 void oracle_scene_free(void* h) { delete static_cast<Scene*>(h); }
+
+// Base-colour textures of a scene: 6 ints per texture (width, height, wrap s,
+// wrap t, filter, first texel) and the RGBA8 texels.
+void oracle_scene_set_textures(void* h, const int32_t* desc, int64_t ntex, const uint32_t* texels, int64_t ntexels) {
+  Scene* S = static_cast<Scene*>(h);
+  S->tex.assign(desc, desc + 6 * (size_t)ntex);
+  S->texels.assign(texels, texels + (size_t)ntexels);
+}
+
+// One texture lookup (texture t at (u, v)) for direct checks of the sampler.
+void oracle_texture_sample(void* h, uint32_t t, float u, float v, float* rgb) {
+  const F3 c = texture_rgb(*static_cast<Scene*>(h), t, u, v);
+  rgb[0] = c.x;
+  rgb[1] = c.y;
+  rgb[2] = c.z;
+}
 
 // info: tris, nodes, stack bound, levels, leaf size, centroid lo xyz/hi xyz as float bits (6)
 void oracle_scene_info(void* h, int64_t* info) {
