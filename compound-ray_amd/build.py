@@ -27,6 +27,7 @@ SOURCES = [
     "scan.hip",
     "lbvh.hip",
     "render.hip",
+    "projection.hip",
     "batch.cpp",
     "mlp.hip",
     "adaptive.hip",

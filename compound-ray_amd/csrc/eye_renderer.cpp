@@ -21,6 +21,7 @@
 #include "device_scene.h"
 #include "mlp.h"
 #include "render.h"
+#include "projection.h"
 #include "scene_loader.h"
 
 using namespace eye;
@@ -53,6 +54,13 @@ struct EyeState {
   size_t d_colours_cap = 0;
   bool has_render = false;
   uint64_t rendered_rays = 0;
+  // pixel -> ommatidium map of the spherical projections, valid for
+  // (proj_w, proj_h, proj_variant) and the ommatidia of proj_version
+  uint32_t* d_proj_ids = nullptr;
+  size_t d_proj_cap = 0;
+  int proj_w = -1, proj_h = -1, proj_variant = -1;
+  uint64_t proj_version = ~0ull;
+  uint64_t omm_version = 0;   // bumped whenever the ommatidia change
 
   uint64_t total_samples() const {
     if (counts.empty()) return (uint64_t)omms.size() * (uint64_t)spp;
@@ -185,6 +193,7 @@ void free_scene() {
     if (e.d_offsets) (void)hipFree(e.d_offsets);
     if (e.d_samples) (void)hipFree(e.d_samples);
     if (e.d_colours) (void)hipFree(e.d_colours);
+    if (e.d_proj_ids) (void)hipFree(e.d_proj_ids);
     if (e.d_counts) (void)hipFree(e.d_counts);
     if (e.d_sample_offsets) (void)hipFree(e.d_sample_offsets);
     if (e.d_order) (void)hipFree(e.d_order);
@@ -331,7 +340,25 @@ double render_compound(EyeState& e, const Cam& c) {
   }
   hip_check(hipEventRecord(s.ev[2], s.stream), "event");
   bool ok = true;
-  if (e.shader == "single_dimension")
+  int sph = -1;
+  bool sph_ids = false;
+  if (e.shader == "spherical_positionwise" || e.shader == "spherical_positionwise_ids") sph = kSphericalPosition;
+  if (e.shader == "spherical_orientationwise" || e.shader == "spherical_orientationwise_ids")
+    sph = kSphericalOrientation;
+  if (e.shader == "spherical_split_orientationwise") sph = kSphericalSplit;
+  if (sph >= 0) sph_ids = e.shader.size() > 4 && e.shader.compare(e.shader.size() - 4, 4, "_ids") == 0;
+  if (sph >= 0) {
+    const int pixels = s.width * s.height;
+    if (e.proj_w != s.width || e.proj_h != s.height || e.proj_variant != sph || e.proj_version != e.omm_version) {
+      ensure_device(&e.d_proj_ids, &e.d_proj_cap, (size_t)std::max(pixels, 1));
+      ok = launch_spherical_ids(e.d_omms, L.omm_count, s.width, s.height, sph, e.d_proj_ids, s.stream, &err);
+      e.proj_w = s.width;
+      e.proj_h = s.height;
+      e.proj_variant = sph;
+      e.proj_version = e.omm_version;
+    }
+    if (ok) ok = launch_frame_lookup(e.d_colours, e.d_proj_ids, pixels, sph_ids, s.d_frame, s.stream, &err);
+  } else if (e.shader == "single_dimension")
     ok = launch_frame_single_dim(e.d_colours, L.omm_count, s.d_frame, s.width, s.height, s.stream, &err);
   else if (e.shader == "raw_ommatidial_samples")
     ok = e.counts.empty() ? launch_frame_raw_samples(e.d_samples, L.omm_count, e.spp, s.d_frame, s.width, s.height,
@@ -697,6 +724,7 @@ void setOmmatidia(OmmatidiumPacket* omms, size_t count) {
     e->offsets.clear();
   }
   e->omms.resize(count);
+  e->omm_version++;
   if (count) memcpy(e->omms.data(), omms, sizeof(Ommatidium) * count);
   e->device_dirty = true;
   ABI_GUARD_END(return)

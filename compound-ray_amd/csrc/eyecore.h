@@ -397,4 +397,36 @@ EYE_HD uint8_t encode_channel(float c) {
   return (uint8_t)(det_pow(cc, kGammaEncode) * 255.0f);
 }
 
+// ---- spherical projections (shaders.cu:412-640) ----
+enum SphericalVariant : int { kSphericalPosition = 0, kSphericalOrientation = 1, kSphericalSplit = 2 };
+
+// Unit-sphere direction shown by pixel (x, y) of a width x height window:
+// longitude pi/2 - pi d.x, latitude pi/2 d.y, d = 2 (x/w, y/h) - 1.  The
+// split variant maps each half of the window onto the whole sphere.
+EYE_HD V3 sphere_direction(int x, int y, int width, int height, bool split) {
+  const float ux = (float)x / (float)width, uy = (float)y / (float)height;
+  float dx, dy;
+  if (split) {
+    const float sx = ux * 2.0f;
+    dx = (sx - (sx > 1.0f ? 1.0f : 0.0f)) * 2.0f - 1.0f;
+    dy = uy * 2.0f - 1.0f;
+  } else {
+    dx = 2.0f * ux - 1.0f;
+    dy = 2.0f * uy - 1.0f;
+  }
+  const float lon = dx * -kPiF + kPiF / 2.0f;
+  const float lat = dy * (kPiF / 2.0f) + 0.0f;
+  float slon, clon, slat, clat;
+  det_sincos(lon, &slon, &clon);
+  det_sincos(lat, &slat, &clat);
+  return v3(clon * clat, slat, slon * clat);
+}
+
+// Cosine of the angle between v and the unit-sphere point s (the reference
+// compares acos of this value; the smallest angle is the largest cosine).
+EYE_HD float cos_angle(V3 v, V3 s, float len_s) {
+  const float len_v = sqrtf((v.x * v.x + v.y * v.y) + v.z * v.z);
+  return ((v.x * s.x + v.y * s.y) + v.z * s.z) / (len_v * len_s);
+}
+
 }  // namespace eye

@@ -52,6 +52,7 @@ def lib():
                                     c_uint32, c_uint32, c_uint32, c_uint32, c_void_p, c_void_p, c_void_p, c_void_p,
                                     c_void_p, c_int]
         L.oracle_make_color.argtypes = [c_void_p, c_int64, c_void_p]
+        L.oracle_spherical_ids.argtypes = [c_void_p, c_int64, c_int, c_int, c_int, c_void_p]
         L.oracle_pose.argtypes = [c_void_p, c_void_p, c_void_p]
         _lib = L
     return _lib
@@ -210,3 +211,12 @@ def mlp_loss_grad(params, X, T=None, mode: int = 0):
     L.oracle_mlp_loss_grad.argtypes = [c_void_p, c_void_p, c_void_p, c_int64, c_int, c_void_p, c_void_p, c_void_p]
     L.oracle_mlp_loss_grad(_p(p), _p(x), _p(t), n, mode, _p(y), _p(grad), _p(loss))
     return y, grad, float(loss[0])
+
+
+def spherical_ids(omm, width: int, height: int, variant: int) -> np.ndarray:
+    """(height, width) ommatidium index per pixel of a spherical projection
+    (0 positionwise, 1 orientationwise, 2 split orientationwise)."""
+    o = np.ascontiguousarray(omm, dtype=np.float32).reshape(-1, 8)
+    out = np.zeros((height, width), np.uint32)
+    lib().oracle_spherical_ids(_p(o), o.shape[0], width, height, variant, _p(out))
+    return out

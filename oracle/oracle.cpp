@@ -1021,4 +1021,48 @@ void oracle_make_color(const float* rgb, int64_t n, uint8_t* out) {
   }
 }
 
+// Spherical projections, libEyeRenderer3/shaders.cu:412-640: for each pixel
+// of a w x h window, the index of the ommatidium whose position (variant 0)
+// or direction (1; 2 = split by the sign of position.x against the pixel's
+// half) makes the smallest angle with the pixel's sphere point.  The angle
+// order is taken on the cosine (acos is decreasing), first index on ties.
+void oracle_spherical_ids(const float* omm, int64_t n, int w, int h, int variant, uint32_t* ids) {
+  for (int y = 0; y < h; ++y)
+    for (int x = 0; x < w; ++x) {
+      const float fx = (float)x / (float)w, fy = (float)y / (float)h;
+      float d0 = 2.0f * fx - 1.0f, d1 = 2.0f * fy - 1.0f;
+      if (variant == 2) {
+        float sx = fx * 2.0f;
+        if (sx > 1.0f) sx = sx - 1.0f;
+        else sx = sx - 0.0f;
+        d0 = sx * 2.0f - 1.0f;
+        d1 = fy * 2.0f - 1.0f;
+      }
+      const float a0 = d0 * -PI_F + PI_F / 2.0f;
+      const float a1 = d1 * (PI_F / 2.0f) + 0.0f;
+      float s0, c0, s1, c1;
+      o_sincos(a0, &s0, &c0);
+      o_sincos(a1, &s1, &c1);
+      const F3 sp = f3(c0 * c1, s1, s0 * c1);
+      const float ls = std::sqrt(dot(sp, sp));
+      uint32_t best_i = 0;
+      float best = 0.0f;
+      for (int64_t i = 0; i < n; ++i) {
+        const float* o = omm + 8 * i;
+        const F3 v = variant == 0 ? f3(o[0], o[1], o[2]) : f3(o[3], o[4], o[5]);
+        const float c = dot(v, sp) / (std::sqrt(dot(v, v)) * ls);
+        if (i == 0) {
+          best = c;
+          continue;
+        }
+        if (variant == 2 && !((o[0] > 0.0f && fx > 0.5f) || (o[0] < 0.0f && fx < 0.5f))) continue;
+        if (c > best) {
+          best = c;
+          best_i = (uint32_t)i;
+        }
+      }
+      ids[(size_t)y * (size_t)w + (size_t)x] = best_i;
+    }
+}
+
 }  // extern "C"

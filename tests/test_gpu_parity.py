@@ -306,3 +306,32 @@ def test_empty_and_tiny_scenes(lib, tmp_path):
         assert np.array_equal(er.bvh_nodes_np(lib), o.nodes())
         want = o.render(eye, er.pose_np(lib), spp=16)["colour_gpu_order"][0]
         assert np.array_equal(er.colours_np(lib), want), ntri
+
+
+@pytest.mark.parametrize("shader,variant,ids", [("spherical_positionwise", 0, False),
+                                                ("spherical_orientationwise", 1, False),
+                                                ("spherical_split_orientationwise", 2, False),
+                                                ("spherical_orientationwise_ids", 1, True),
+                                                ("spherical_positionwise_ids", 0, True)])
+def test_spherical_projections(lib, config1, oracle_c1, shader, variant, ids):
+    """Window pixels show the ommatidium closest in angle (oracle ids), as its
+    make_color colour or as its index in the RGBA bytes."""
+    omm = config1["eyes"][0][:500]
+    setup_eye(lib, config1, omm=omm, spp=8, record=False)
+    lib.setCurrentEyeShaderName(shader.encode())
+    w, h = 72, 36
+    er.setRenderSize(lib, w, h)
+    try:
+        for _ in range(2):   # second frame reuses the cached pixel map
+            assert lib.renderFrame() >= 0, er.last_error(lib)
+            frame = np.array(lib.getFramePointer(), copy=True).reshape(h, w, 4)
+            want_ids = oracle.spherical_ids(omm, w, h, variant)
+            if ids:
+                enc = np.stack([want_ids >> 24, (want_ids >> 16) & 255, (want_ids >> 8) & 255, want_ids & 255], -1)
+                assert np.array_equal(frame, enc.astype(np.uint8))
+            else:
+                col = oracle.make_color(er.colours_np(lib))
+                assert np.array_equal(frame, col[want_ids])
+    finally:
+        lib.setCurrentEyeShaderName(b"single_dimension_fast")
+        er.setRenderSize(lib, 400, 400)

@@ -431,3 +431,52 @@ def test_mlp_bf16_mode_close_to_exact():
     y1, g1, _ = oracle.mlp_loss_grad(p, x, t, mode=1)
     assert np.max(np.abs(y1 - y0)) < 0.05 * np.max(np.abs(y0))
     assert np.linalg.norm(g1 - g0) < 0.05 * np.linalg.norm(g0)
+
+
+def test_spherical_ids_closed_forms():
+    """Spherical projections (shaders.cu:412-640): the six axis ommatidia own
+    the pixels looking along their axes; on random eyes the chosen ommatidium
+    maximises the float64 cosine (up to near-ties); the split variant keeps
+    each half of the window to ommatidia on its side of x = 0."""
+    axes = np.array([[1, 0, 0], [-1, 0, 0], [0, 1, 0], [0, -1, 0], [0, 0, 1], [0, 0, -1]], np.float32)
+    omm = np.zeros((6, 8), np.float32)
+    omm[:, 0:3] = axes * 0.001
+    omm[:, 3:6] = axes
+    w, h = 8, 4
+    for variant in (0, 1):
+        ids = oracle.spherical_ids(omm, w, h, variant)
+        # d = 2 (x/w, y/h) - 1; lon = pi/2 - pi d.x; lat = pi/2 d.y
+        assert ids[2, 4] == 4            # d = (0, 0): lon = pi/2 -> (0, 0, 1)
+        assert ids[2, 6] == 0            # d.x = 1/2: lon = 0 -> (1, 0, 0)
+        assert ids[2, 0] == 5            # d.x = -1: lon = 3 pi/2 -> (0, 0, -1)
+        assert ids[0, 4] == 3            # d.y = -1: lat = -pi/2 -> (0, -1, 0)
+    rng = np.random.default_rng(4)
+    d = rng.normal(size=(300, 3))
+    omm = np.zeros((300, 8), np.float32)
+    omm[:, 3:6] = d
+    omm[:, 0:3] = d * 0.001
+    w, h = 40, 20
+    ids = oracle.spherical_ids(omm, w, h, 1)
+    ys, xs = np.mgrid[0:h, 0:w]
+    lon = (2 * xs / w - 1) * -np.pi + np.pi / 2
+    lat = (2 * ys / h - 1) * np.pi / 2
+    s = np.stack([np.cos(lon) * np.cos(lat), np.sin(lat), np.sin(lon) * np.cos(lat)], -1)
+    dn = d / np.linalg.norm(d, axis=1, keepdims=True)
+    cos = s.reshape(-1, 3) @ dn.T
+    want = cos.argmax(axis=1)
+    got = ids.reshape(-1)
+    agree = got == want
+    assert agree.mean() > 0.995
+    gap = cos[np.arange(len(got)), want] - cos[np.arange(len(got)), got]
+    assert np.all(gap[~agree] < 1e-5)
+    # split: left half only ommatidia with x < 0 (index 0 is the initial pick)
+    sids = oracle.spherical_ids(omm, w, h, 2)
+    for y in range(h):
+        for x in range(w):
+            i = sids[y, x]
+            if i == 0:
+                continue
+            if x / w < 0.5:
+                assert omm[i, 0] < 0
+            elif x / w > 0.5:
+                assert omm[i, 0] > 0
