@@ -75,6 +75,11 @@ struct Cam {
   Pose pose;
   int eye = -1;
   float yfov_deg = 60.0f;
+  // pinhole: (tan(yfov/2), tan(yfov/2), 1), or (10, 10, 1) for the default
+  // camera, which never gets a field of view (PerspectiveCamera.cpp:3-7);
+  // orthographic: (xmag, ymag, -)
+  float scale[3] = {10.0f, 10.0f, 1.0f};
+  float start_radius = 0.0f;   // panoramic (PanoramicCamera.cpp:16)
 };
 
 struct State {
@@ -383,6 +388,42 @@ double render_compound(EyeState& e, const Cam& c) {
   return ms;
 }
 
+CameraLaunch camera_launch_of(const Cam& c) {
+  State& s = st();
+  CameraLaunch C;
+  C.pose = c.pose;
+  C.kind = c.kind == kOrthographic ? kCameraOrthographic : c.kind == kPanoramic ? kCameraPanoramic : kCameraPinhole;
+  C.scale_x = c.scale[0];
+  C.scale_y = c.scale[1];
+  C.scale_z = c.scale[2];
+  C.start_radius = c.start_radius;
+  C.width = s.width;
+  C.height = s.height;
+  return C;
+}
+
+// Pinhole, orthographic and panoramic cameras: one primary ray per pixel.
+double render_regular(const Cam& c) {
+  State& s = st();
+  ensure_frame_buffer();
+  std::string err;
+  const auto t0 = std::chrono::steady_clock::now();
+  hip_check(hipEventRecord(s.ev[0], s.stream), "event");
+  if (!launch_camera_frame(s.scene, camera_launch_of(c), s.d_frame, nullptr, s.stream, &err))
+    throw std::runtime_error(err);
+  hip_check(hipEventRecord(s.ev[1], s.stream), "event");
+  hip_check(hipStreamSynchronize(s.stream), "render");
+  const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  float f = 0.0f;
+  (void)hipEventElapsedTime(&f, s.ev[0], s.ev[1]);
+  s.timings[0] = f;
+  s.timings[1] = 0.0;
+  s.timings[2] = 0.0;
+  s.timings[3] = f;
+  s.frame_host_stale = true;
+  return ms;
+}
+
 void recompute_offsets(EyeState& e) {
   e.offsets.clear();
   if (e.counts.empty()) return;
@@ -484,6 +525,15 @@ void loadGlTFscene(const char* filepath) {
     c.kind = cd.kind;
     c.pose = cd.pose;
     c.yfov_deg = cd.yfov_deg;
+    if (cd.kind == kPerspective) {
+      // setYFOV (PerspectiveCamera.cpp:19-24) with aspect ratio 1
+      const float yfov = cd.yfov_deg / 180.0f * kPiF;
+      c.scale[1] = std::tan(yfov / 2.0f) * c.scale[2];
+      c.scale[0] = c.scale[1] * 1.0f;
+    } else if (cd.kind == kOrthographic) {
+      c.scale[0] = cd.xmag;
+      c.scale[1] = cd.ymag;
+    }
     if (cd.kind == kCompound) {
       EyeState e;
       e.omms = std::move(cd.ommatidia);
@@ -550,8 +600,8 @@ double renderFrame(void) {
   double ms = 0.0;
   if (c && c->eye >= 0) {
     ms = render_compound(s.eyes[(size_t)c->eye], *c);
-  } else {
-    throw std::runtime_error("renderFrame: regular (non-compound) cameras are not rendered by this library");
+  } else if (c) {
+    ms = render_regular(*c);
   }
   s.last_error.clear();
   if (s.verbose) std::cout << "[PyEye] Rendered frame in " << ms << "ms." << std::endl;
@@ -1323,6 +1373,24 @@ int64_t parseGlTFTriangles(const char* path, float* pos, float* col, uint32_t* m
   }
   return (int64_t)n;
   ABI_GUARD_END(return -1)
+}
+
+size_t getCurrentCameraRays(float* rays, size_t maxRays) {
+  ABI_GUARD_BEGIN
+  State& s = st();
+  Cam* c = current_cam();
+  if (!s.loaded || !c || c->eye >= 0) return 0;
+  const size_t n = (size_t)std::max(s.width, 0) * (size_t)std::max(s.height, 0);
+  if (!rays || maxRays == 0) return n;
+  DeviceTemp tmp;
+  Ray* d = tmp.upload<Ray>(nullptr, n);
+  std::string err;
+  if (!launch_camera_frame(s.scene, camera_launch_of(*c), nullptr, d, s.stream, &err)) throw std::runtime_error(err);
+  hip_check(hipStreamSynchronize(s.stream), "camera rays");
+  const size_t m = std::min(n, maxRays);
+  hip_check(hipMemcpy(rays, d, sizeof(Ray) * m, hipMemcpyDeviceToHost), "readback");
+  return m;
+  ABI_GUARD_END(return 0)
 }
 
 int64_t parseGlTFTextures(const char* path, int32_t* desc, size_t maxTextures, uint32_t* texels, size_t maxTexels,

@@ -397,6 +397,17 @@ EYE_HD uint8_t encode_channel(float c) {
   return (uint8_t)(det_pow(cc, kGammaEncode) * 255.0f);
 }
 
+// ---- regular cameras (shaders.cu:198-333) ----
+enum CameraRayKind : int { kCameraPinhole = 0, kCameraOrthographic = 1, kCameraPanoramic = 2 };
+
+struct CameraLaunch {
+  Pose pose;
+  int kind;                   // CameraRayKind
+  float scale_x, scale_y, scale_z;  // pinhole: tan(fov/2) x, y and 1; orthographic: xmag, ymag
+  float start_radius;         // panoramic
+  int width, height;
+};
+
 // ---- spherical projections (shaders.cu:412-640) ----
 enum SphericalVariant : int { kSphericalPosition = 0, kSphericalOrientation = 1, kSphericalSplit = 2 };
 
@@ -420,6 +431,37 @@ EYE_HD V3 sphere_direction(int x, int y, int width, int height, bool split) {
   det_sincos(lon, &slon, &clon);
   det_sincos(lat, &slat, &clat);
   return v3(clon * clat, slat, slon * clat);
+}
+
+// Primary ray of pixel (x, y) of a regular camera, d = 2 (x/w, y/h) - 1:
+// pinhole: direction Z sz + d.x X sx + d.y Y sy from the position (not
+// normalised); orthographic: direction Z from position + d.x X sx + d.y Y sy;
+// panoramic: the unit direction of sphere_direction in the camera's axes,
+// from position + direction * start radius.  tmin 0.01, tmax 1e16.
+EYE_HD Ray camera_ray(const CameraLaunch& C, int x, int y) {
+  const Pose& P = C.pose;
+  const V3 X = v3(P.x[0], P.x[1], P.x[2]), Y = v3(P.y[0], P.y[1], P.y[2]), Z = v3(P.z[0], P.z[1], P.z[2]);
+  const V3 O = v3(P.pos[0], P.pos[1], P.pos[2]);
+  const float dx = 2.0f * ((float)x / (float)C.width) - 1.0f;
+  const float dy = 2.0f * ((float)y / (float)C.height) - 1.0f;
+  V3 o, d;
+  if (C.kind == kCameraOrthographic) {
+    d = Z;
+    o = (O + (dx * X) * C.scale_x) + (dy * Y) * C.scale_y;
+  } else if (C.kind == kCameraPanoramic) {
+    const V3 s = sphere_direction(x, y, C.width, C.height, false);
+    d = normalize((s.x * X + s.y * Y) + s.z * Z);
+    o = O + d * C.start_radius;
+  } else {
+    d = (Z * C.scale_z + (dx * X) * C.scale_x) + (dy * Y) * C.scale_y;
+    o = O;
+  }
+  Ray r;
+  r.ox = o.x; r.oy = o.y; r.oz = o.z;
+  r.tmin = 0.01f;
+  r.dx = d.x; r.dy = d.y; r.dz = d.z;
+  r.tmax = 1e16f;
+  return r;
 }
 
 // Cosine of the angle between v and the unit-sphere point s (the reference

@@ -103,6 +103,12 @@ bool launch_frame_single_dim(const float4* colours, int omm_count, uchar4* frame
 bool launch_frame_raw_samples(const float4* samples, int omm_count, int spp, uchar4* frame, int width,
                               int height, hipStream_t stream, std::string* err);
 
+// Frame of a regular camera (pinhole, orthographic or panoramic,
+// shaders.cu:198-333): every pixel traces its primary ray and shows
+// make_color of the hit's shading or the sky.  frame and rays are nullable.
+bool launch_camera_frame(const DeviceScene& scene, const CameraLaunch& C, uchar4* frame, Ray* rays,
+                         hipStream_t stream, std::string* err);
+
 // Rays of one eye at one pose, in (ommatidium, sample) order, 8 floats each.
 bool launch_generate_rays(const EyeLaunch& L, Ray* rays, hipStream_t stream, std::string* err);
 

@@ -545,6 +545,24 @@ __device__ __forceinline__ uchar4 make_color(float4 c) {
   return make_uchar4(encode_channel(c.x), encode_channel(c.y), encode_channel(c.z), 255);
 }
 
+// One pixel of a regular camera per thread: primary ray, closest hit
+// (per-ray LDS stack), shading or sky, make_color.  rays (nullable) receives
+// the generated rays.
+__global__ __launch_bounds__(kTraceThreads) void k_camera_frame(SceneView S, CameraLaunch C,
+                                                                uchar4* __restrict__ frame, Ray* __restrict__ rays) {
+  extern __shared__ uint32_t lds_stack[];
+  const int pixels = C.width * C.height;
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= pixels) return;
+  const Ray r = camera_ray(C, p % C.width, p / C.width);
+  if (rays) rays[p] = r;
+  if (!frame) return;
+  Hit h;
+  closest_hit(S, precompute_ray(r), lds_stack + threadIdx.x, blockDim.x, h);
+  const V3 c = shade(S, h, r.dx, r.dy, r.dz);
+  frame[p] = make_color(make_float4(c.x, c.y, c.z, 0.0f));
+}
+
 __global__ void k_frame_sdf(const float4* __restrict__ colours, int omm_count, uchar4* __restrict__ frame,
                             int width) {
   const int x = blockIdx.x * blockDim.x + threadIdx.x;
@@ -747,6 +765,24 @@ bool launch_frame_raw_samples(const float4* samples, int omm_count, int spp, uch
   hipLaunchKernelGGL(k_frame_raw, dim3((width + 255) / 256, height), dim3(256), 0, stream, samples, omm_count, spp,
                      frame, width, height);
   return check_launch("frame", err);
+}
+
+bool launch_camera_frame(const DeviceScene& scene, const CameraLaunch& C, uchar4* frame, Ray* rays,
+                         hipStream_t stream, std::string* err) {
+  if (C.width <= 0 || C.height <= 0) return true;
+  const long long pixels = (long long)C.width * C.height;
+  if (pixels > 0x7fffffffLL) {
+    if (err) *err = "camera: frame too large";
+    return false;
+  }
+  const size_t lds = stack_bytes(scene);
+  if (lds > 160 * 1024) {
+    if (err) *err = "camera: BVH too deep for the LDS traversal stack";
+    return false;
+  }
+  hipLaunchKernelGGL(k_camera_frame, dim3((unsigned)((pixels + kTraceThreads - 1) / kTraceThreads)),
+                     dim3(kTraceThreads), lds, stream, view_of(scene), C, frame, rays);
+  return check_launch("camera_frame", err);
 }
 
 bool launch_generate_rays(const EyeLaunch& L, Ray* rays, hipStream_t stream, std::string* err) {

@@ -163,6 +163,8 @@ def configure_extensions(lib):
     lib.setSceneLeafSize.restype = c_int
     lib.parseGlTFTriangles.argtypes = [c_char_p, c_void_p, c_void_p, c_void_p, c_size_t]
     lib.parseGlTFTriangles.restype = c_int64
+    lib.getCurrentCameraRays.argtypes = [c_void_p, c_size_t]
+    lib.getCurrentCameraRays.restype = c_size_t
     lib.parseGlTFTextures.argtypes = [c_char_p, c_void_p, c_size_t, c_void_p, c_size_t, c_void_p]
     lib.parseGlTFTextures.restype = c_int64
 
@@ -415,6 +417,15 @@ def parse_gltf_np(lib, path: str):
     lib.parseGlTFTriangles(path.encode(), pos.ctypes.data_as(c_void_p), col.ctypes.data_as(c_void_p),
                            mode.ctypes.data_as(c_void_p), n)
     return pos, col, mode
+
+
+def camera_rays_np(lib) -> np.ndarray:
+    """(h * w, 8) primary rays of the current regular camera."""
+    n = lib.getCurrentCameraRays(None, 0)
+    out = np.zeros((n, 8), np.float32)
+    if n:
+        lib.getCurrentCameraRays(out.ctypes.data_as(c_void_p), n)
+    return out
 
 
 def parse_textures_np(lib, path: str):

@@ -134,9 +134,23 @@ class Eye:
 
 
 @dataclass
+class Camera:
+    """Regular glTF camera: "perspective" (yfov radians), "orthographic"
+    (xmag, ymag) or "panoramic" (a perspective camera with the panoramic extra)."""
+    name: str
+    kind: str = "perspective"
+    yfov: float = 1.0
+    xmag: float = 1.0
+    ymag: float = 1.0
+    translation: tuple = (0.0, 0.0, 0.0)
+    yaw: float = 0.0
+
+
+@dataclass
 class Scene:
     meshes: list
     eyes: list = field(default_factory=list)
+    cameras: list = field(default_factory=list)   # regular cameras, listed after the eyes
     background: str = "simple_sky"
     heightfield: tuple | None = None   # (xs, zs, heights) for ground queries
 
@@ -406,6 +420,18 @@ def write_gltf(directory, name: str, scene: Scene) -> pathlib.Path:
                                    "compound-projection": e.projection}})
         nodes.append({"name": e.name, "camera": len(cameras) - 1, "translation": [float(v) for v in e.translation],
                       "rotation": [0.0, math.sin(e.yaw / 2), 0.0, math.cos(e.yaw / 2)]})
+    for c in scene.cameras:
+        if c.kind == "orthographic":
+            cam = {"name": c.name, "type": "orthographic",
+                   "orthographic": {"xmag": c.xmag, "ymag": c.ymag, "znear": 0.01, "zfar": 100.0}}
+        else:
+            cam = {"name": c.name, "type": "perspective",
+                   "perspective": {"yfov": c.yfov, "znear": 0.01, "aspectRatio": 1.0}}
+            if c.kind == "panoramic":
+                cam["extras"] = {"panoramic": "true"}
+        cameras.append(cam)
+        nodes.append({"name": c.name, "camera": len(cameras) - 1, "translation": [float(v) for v in c.translation],
+                      "rotation": [0.0, math.sin(c.yaw / 2), 0.0, math.cos(c.yaw / 2)]})
     bin_name = f"{name}.bin"
     (d / bin_name).write_bytes(bytes(blob))
     doc = {

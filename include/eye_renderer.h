@@ -235,6 +235,11 @@ int autotuneRenderBatch(const float* poses, size_t poseCount, int repeats, float
  * (k_copy_probe), `repeats` times; returns the mean GPU ms per copy or -1. */
 double runCopyProbe(size_t bytes, int repeats);
 
+/* Primary rays (8 floats: origin, tmin, direction, tmax) of every pixel of the
+ * current regular camera at its pose and the render size, row-major; returns
+ * the number written (maxRays = 0: the number of pixels), 0 for a compound eye. */
+size_t getCurrentCameraRays(float* rays, size_t maxRays);
+
 /* Host-only glTF parse (no GPU): world-space triangles (9 floats), colours
  * (9 floats) and colour modes of `path`.  Returns the triangle count (call
  * with maxTris = 0 to query), or -1 on error. */

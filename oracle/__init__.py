@@ -53,6 +53,8 @@ def lib():
                                     c_void_p, c_int]
         L.oracle_make_color.argtypes = [c_void_p, c_int64, c_void_p]
         L.oracle_spherical_ids.argtypes = [c_void_p, c_int64, c_int, c_int, c_int, c_void_p]
+        L.oracle_camera_rays.argtypes = [c_int, c_void_p, c_void_p, c_float, c_int, c_int, c_void_p]
+        L.oracle_radiance.argtypes = [c_void_p, c_void_p, c_int64, c_void_p, c_int]
         L.oracle_pose.argtypes = [c_void_p, c_void_p, c_void_p]
         _lib = L
     return _lib
@@ -119,6 +121,13 @@ class OracleScene:
     def from_scene(cls, scene, leaf_size: int = 4):
         return cls(scene.tri_pos(), scene.tri_col(), scene.tri_mode(), scene.sky_mode(), leaf_size,
                    textures=scene.texture_arrays())
+
+    def radiance(self, rays, threads: int | None = None) -> np.ndarray:
+        """(n, 3) shaded colour (or sky) of each ray's closest hit."""
+        r = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 8)
+        out = np.zeros((r.shape[0], 3), np.float32)
+        lib().oracle_radiance(self.h, _p(r), r.shape[0], _p(out), threads or default_threads())
+        return out
 
     def texture_sample(self, tex: int, u: float, v: float) -> np.ndarray:
         """One lookup of texture `tex` at (u, v): linear RGB before linearize."""
@@ -219,4 +228,14 @@ def spherical_ids(omm, width: int, height: int, variant: int) -> np.ndarray:
     o = np.ascontiguousarray(omm, dtype=np.float32).reshape(-1, 8)
     out = np.zeros((height, width), np.uint32)
     lib().oracle_spherical_ids(_p(o), o.shape[0], width, height, variant, _p(out))
+    return out
+
+
+def camera_rays(kind: int, pose12, scale, width: int, height: int, start_radius: float = 0.0) -> np.ndarray:
+    """(h * w, 8) primary rays of a regular camera (0 pinhole, 1 orthographic,
+    2 panoramic)."""
+    P = np.ascontiguousarray(pose12, dtype=np.float32).reshape(12)
+    sc = np.ascontiguousarray(scale, dtype=np.float32).reshape(3)
+    out = np.zeros((height * width, 8), np.float32)
+    lib().oracle_camera_rays(kind, _p(P), _p(sc), start_radius, width, height, _p(out))
     return out

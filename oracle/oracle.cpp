@@ -1021,6 +1021,55 @@ void oracle_make_color(const float* rgb, int64_t n, uint8_t* out) {
   }
 }
 
+// Primary rays of the regular cameras, libEyeRenderer3/shaders.cu:198-333.
+// pose: position, x, y, z axes (12 floats); kind 0 pinhole (__raygen__pinhole:
+// direction z*sz + d.x*x*sx + d.y*y*sy, unnormalised), 1 orthographic
+// (direction z, origin shifted by d.x*x*sx + d.y*y*sy), 2 panoramic (unit
+// sphere direction in the camera axes, origin moved out by start_radius).
+void oracle_camera_rays(int kind, const float* pose, const float* sc, float start_radius, int w, int h,
+                        float* out) {
+  const F3 O = f3(pose[0], pose[1], pose[2]), X = f3(pose[3], pose[4], pose[5]);
+  const F3 Y = f3(pose[6], pose[7], pose[8]), Z = f3(pose[9], pose[10], pose[11]);
+  for (int y = 0; y < h; ++y)
+    for (int x = 0; x < w; ++x) {
+      const float fx = (float)x / (float)w, fy = (float)y / (float)h;
+      const float d0 = 2.0f * fx - 1.0f, d1 = 2.0f * fy - 1.0f;
+      F3 o, d;
+      if (kind == 1) {
+        d = Z;
+        o = add(add(O, scale(scale(X, d0), sc[0])), scale(scale(Y, d1), sc[1]));
+      } else if (kind == 2) {
+        const float a0 = d0 * -PI_F + PI_F / 2.0f, a1 = d1 * (PI_F / 2.0f) + 0.0f;
+        float s0, c0, s1, c1;
+        o_sincos(a0, &s0, &c0);
+        o_sincos(a1, &s1, &c1);
+        const F3 u = f3(c0 * c1, s1, s0 * c1);
+        d = normalize(add(add(scale(X, u.x), scale(Y, u.y)), scale(Z, u.z)));
+        o = add(O, scale(d, start_radius));
+      } else {
+        d = add(add(scale(Z, sc[2]), scale(scale(X, d0), sc[0])), scale(scale(Y, d1), sc[1]));
+        o = O;
+      }
+      float* r = out + 8 * ((size_t)y * (size_t)w + (size_t)x);
+      r[0] = o.x; r[1] = o.y; r[2] = o.z; r[3] = 0.01f;
+      r[4] = d.x; r[5] = d.y; r[6] = d.z; r[7] = 1e16f;
+    }
+}
+
+// Radiance of arbitrary rays: closest hit (BVH) shaded, or the sky.
+void oracle_radiance(void* h, const float* rays, int64_t n, float* rgb, int threads) {
+  const Scene& S = *static_cast<Scene*>(h);
+  parallel_for((size_t)n, threads, [&](size_t k) {
+    RayO r;
+    std::memcpy(&r, rays + 8 * k, sizeof(RayO));
+    const HitO hh = trace_bvh(S, r, nullptr, nullptr);
+    const F3 c = shade(S, hh, r);
+    rgb[3 * k] = c.x;
+    rgb[3 * k + 1] = c.y;
+    rgb[3 * k + 2] = c.z;
+  });
+}
+
 // Spherical projections, libEyeRenderer3/shaders.cu:412-640: for each pixel
 // of a w x h window, the index of the ommatidium whose position (variant 0)
 // or direction (1; 2 = split by the sign of position.x against the pixel's

@@ -480,3 +480,29 @@ def test_spherical_ids_closed_forms():
                 assert omm[i, 0] < 0
             elif x / w > 0.5:
                 assert omm[i, 0] > 0
+
+
+def test_camera_rays_closed_forms():
+    """Regular cameras (shaders.cu:198-333) on an identity pose: the pinhole's
+    centre pixel looks along z and its corners along z - x sx - y sy; the
+    orthographic camera's rays are parallel to z from a grid spanning +-sx,
+    +-sy; panoramic directions are unit vectors, the centre pixel along z and
+    a quarter of the width to the left along -x."""
+    pose = np.array([1, 2, 3, 1, 0, 0, 0, 1, 0, 0, 0, 1], np.float32)
+    w, h = 8, 4
+    sc = np.array([0.5, 0.25, 1.0], np.float32)
+    pin = oracle.camera_rays(0, pose, sc, w, h).reshape(h, w, 8)
+    assert np.array_equal(pin[2, 4, 4:7], [0, 0, 1])
+    assert np.array_equal(pin[0, 0, 4:7], [-0.5, -0.25, 1])
+    assert np.all(pin[..., 0:3] == [1, 2, 3]) and np.all(pin[..., 3] == np.float32(0.01))
+    assert np.all(pin[..., 7] == np.float32(1e16))
+    ortho = oracle.camera_rays(1, pose, np.array([2.0, 1.0, 0.0], np.float32), w, h).reshape(h, w, 8)
+    assert np.all(ortho[..., 4:7] == [0, 0, 1])
+    assert np.array_equal(ortho[0, 0, 0:3], [1 - 2, 2 - 1, 3])
+    assert np.allclose(ortho[:, :, 0], 1 + 2 * (2 * np.arange(w) / w - 1)[None, :])
+    pano = oracle.camera_rays(2, pose, sc, w, h).reshape(h, w, 8)
+    assert np.allclose(np.linalg.norm(pano[..., 4:7], axis=-1), 1.0, atol=1e-6)
+    assert np.allclose(pano[2, 4, 4:7], [0, 0, 1], atol=1e-6)
+    assert np.allclose(pano[2, 6, 4:7], [1, 0, 0], atol=1e-6)    # lon = 0
+    assert np.allclose(pano[2, 2, 4:7], [-1, 0, 0], atol=1e-6)   # lon = pi
+    assert np.all(pano[..., 0:3] == [1, 2, 3])
