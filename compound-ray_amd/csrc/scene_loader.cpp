@@ -406,6 +406,25 @@ struct Loader {
     outp[0] = o[0]; outp[1] = o[1]; outp[2] = o[2];
   }
 
+  // World box of an object-space box: the box around its 8 transformed
+  // corners (sutil Aabb::transform, used by MulticamScene.cpp:388-389 and
+  // hitscanprocessing.cpp:101-105).
+  void transform_box(const Mat4& xf, const float lo[3], const float hi[3], float wlo[3], float whi[3]) {
+    for (int a = 0; a < 3; ++a) {
+      wlo[a] = INFINITY;
+      whi[a] = -INFINITY;
+    }
+    for (int c = 0; c < 8; ++c) {
+      const float v[4] = {(c & 4) ? hi[0] : lo[0], (c & 2) ? hi[1] : lo[1], (c & 1) ? hi[2] : lo[2], 1.0f};
+      float o[4];
+      apply(xf, v, o);
+      for (int a = 0; a < 3; ++a) {
+        wlo[a] = std::min(wlo[a], o[a]);
+        whi[a] = std::max(whi[a], o[a]);
+      }
+    }
+  }
+
   bool camera(const JsonValue& cam, const Mat4& xf) {
     CameraDesc cd;
     cd.name = cam["name"].as_string();
@@ -461,7 +480,7 @@ struct Loader {
     if (extras_true(m["extras"], "hitbox")) {
       HitboxMesh hb;
       hb.name = name;
-      for (int a = 0; a < 3; ++a) { hb.lo[a] = INFINITY; hb.hi[a] = -INFINITY; }
+      float olo[3] = {INFINITY, INFINITY, INFINITY}, ohi[3] = {-INFINITY, -INFINITY, -INFINITY};
       const JsonValue& prims = m["primitives"];
       for (size_t pi = 0; pi < prims.size(); ++pi) {
         const JsonValue& prim = prims[pi];
@@ -480,10 +499,16 @@ struct Loader {
             transform_point(xf, ident, p, w);
             for (int a = 0; a < 3; ++a) {
               hb.tris.push_back(w[a]);
-              hb.lo[a] = std::min(hb.lo[a], w[a]);
-              hb.hi[a] = std::max(hb.hi[a], w[a]);
+              olo[a] = std::min(olo[a], p[a]);
+              ohi[a] = std::max(ohi[a], p[a]);
             }
           }
+      }
+      // object-space box of the vertices, then its transformed corners
+      if (olo[0] <= ohi[0]) {
+        transform_box(xf, olo, ohi, hb.lo, hb.hi);
+      } else {
+        for (int a = 0; a < 3; ++a) hb.lo[a] = hb.hi[a] = 0.0f;
       }
       out->hitboxes.push_back(std::move(hb));
       return true;
@@ -603,6 +628,21 @@ struct Loader {
         if (tex >= 0 && !use_vc)
           for (int z = 0; z < 3; ++z) out->col.push_back(0.0f);
         out->mode.push_back(mode_word);
+      }
+      // World box from the POSITION accessor's declared min/max when present
+      // (MulticamScene.cpp:377-389), else from the vertices.
+      const JsonValue& pacc = doc["accessors"][(size_t)attrs["POSITION"].as_int(-1)];
+      if (pacc["min"].size() >= 3 && pacc["max"].size() >= 3) {
+        float olo[3], ohi[3], wlo[3], whi[3];
+        for (int a = 0; a < 3; ++a) {
+          olo[a] = (float)pacc["min"][(size_t)a].num;
+          ohi[a] = (float)pacc["max"][(size_t)a].num;
+        }
+        transform_box(xf, olo, ohi, wlo, whi);
+        for (int a = 0; a < 3; ++a) {
+          mb[(size_t)a] = wlo[a];
+          mb[(size_t)a + 3] = whi[a];
+        }
       }
       out->mesh_bounds.push_back(mb);
     }
