@@ -365,6 +365,39 @@ __device__ __forceinline__ uint32_t unit_count(const UnitLaunch& U, uint64_t u, 
   return (uint32_t)U.spp;
 }
 
+// Segmented form of wave_sum: lanes [a, a + seg) with a a multiple of seg
+// (a power of two) sum their values; lane a ends with the total.  For a unit
+// whose samples occupy lanes a .. a + n - 1 (n <= seg, other lanes 0) this is
+// bit-identical to wave_sum over a wave holding the same samples in lanes
+// 0 .. n - 1: the butterfly steps with offset >= seg only add zeros there.
+__device__ __forceinline__ float segment_sum(float v, int seg) {
+  for (int m = 32; m >= 1; m >>= 1) {
+    const float w = __shfl_xor(v, m);
+    if (m < seg) v = v + w;
+  }
+  return v;
+}
+
+// Pose, ommatidium and output slot of queue position u.
+__device__ __forceinline__ void locate_unit(const UnitLaunch& U, uint64_t u, int* p, int* i, uint64_t* out) {
+  *p = (int)(u / (uint64_t)U.omm_count);
+  const int k = (int)(u - (uint64_t)*p * (uint64_t)U.omm_count);
+  *i = U.order ? (int)U.order[k] : k;
+  *out = (uint64_t)*p * (uint64_t)U.omm_count + (uint64_t)*i;
+}
+
+__device__ __forceinline__ int pow2_at_least(uint32_t n) {
+  int s = 1;
+  while ((uint32_t)s < n) s <<= 1;
+  return s;
+}
+
+// Persistent unit renderer.  A wave takes units_per_grab consecutive units
+// from the queue and renders them in rounds: a unit of more than 32 samples
+// fills the wave alone (lane l takes samples l, l + 64, ...); smaller units are
+// packed side by side into power-of-two lane segments, so the lanes of a
+// wave stay busy when sample counts are small (adaptive pilots, uneven
+// counts).  Each unit's colour is the same canonical sum either way.
 template <bool kStats, bool kSamples, bool kScalar>
 __global__ __launch_bounds__(kTraceThreads) void k_render_units(SceneView S, UnitLaunch U, int wstride) {
   extern __shared__ int wave_stacks[];
@@ -376,26 +409,59 @@ __global__ __launch_bounds__(kTraceThreads) void k_render_units(SceneView S, Uni
     if (lane == 0) base = atomicAdd(U.queue, (uint32_t)U.units_per_grab);
     base = __builtin_amdgcn_readfirstlane(__shfl(base, 0));
     if ((uint64_t)base >= units) break;
-    for (int g = 0; g < U.units_per_grab; ++g) {
-      const uint64_t u = (uint64_t)base + (uint64_t)g;
-      if (u >= units) break;
-      const int p = (int)(u / (uint64_t)U.omm_count);
-      const int k = (int)(u - (uint64_t)p * (uint64_t)U.omm_count);
-      const int i = U.order ? (int)U.order[k] : k;
-      const uint64_t out = (uint64_t)p * (uint64_t)U.omm_count + (uint64_t)i;
-      const uint32_t n = unit_count(U, out, i);
+    const int G = (int)min((uint64_t)U.units_per_grab, units - (uint64_t)base);
+    // lane g < G holds the sample count of unit base + g
+    uint32_t my_n = 0;
+    if (lane < G) {
+      int p, i;
+      uint64_t out;
+      locate_unit(U, (uint64_t)base + (uint64_t)lane, &p, &i, &out);
+      my_n = unit_count(U, out, i);
+    }
+    int g = 0;
+    while (g < G) {
+      // ---- build a round: which unit (slot) each lane serves ----
+      int slot = -1, seg_off = 0, seg = 64;
+      int g_end = g + 1;
+      const uint32_t n_first = (uint32_t)__builtin_amdgcn_readlane((int)my_n, g);
+      if (n_first > 32u) {
+        slot = g;
+      } else {
+        int cursor = 0;
+        g_end = g;
+        while (g_end < G) {
+          const uint32_t nj = (uint32_t)__builtin_amdgcn_readlane((int)my_n, g_end);
+          if (nj > 32u) break;
+          const int sj = pow2_at_least(nj);
+          const int a = (cursor + sj - 1) & ~(sj - 1);
+          if (a + sj > 64) break;
+          if (lane >= a && lane < a + sj) {
+            slot = g_end;
+            seg_off = a;
+            seg = sj;
+          }
+          cursor = a + sj;
+          ++g_end;
+        }
+      }
+      // ---- this lane's unit ----
+      const int src = slot < 0 ? 0 : slot;
+      int p, i;
+      uint64_t out;
+      locate_unit(U, (uint64_t)base + (uint64_t)src, &p, &i, &out);
+      const uint32_t n = slot < 0 ? 0u : (uint32_t)__shfl((int)my_n, src);
       const float inv = 1.0f / (float)(n > 0u ? n : 1u);
-      const Ommatidium o = U.omms[i];
-      const Pose pose = U.poses[p];
+      const uint32_t s_lane = (uint32_t)(lane - seg_off);
       float r = 0.0f, gg = 0.0f, b = 0.0f;
       float sr = 0.0f, sg = 0.0f, sb = 0.0f, sd = 0.0f, sd2 = 0.0f, sh = 0.0f;
-      for (uint32_t c0 = 0; c0 < n; c0 += 64u) {
-        const uint32_t s = c0 + (uint32_t)lane;
-        const bool active = s < n;
+      for (uint32_t c0 = 0; __ballot(slot >= 0 && c0 + s_lane < n) != 0ull; c0 += 64u) {
+        const uint32_t s = c0 + s_lane;
+        const bool active = slot >= 0 && s < n;
         Ray ray;
         if (active) {
-          const U4 rnd = sample_randoms(U.seed, U.eye, (uint32_t)i, U.sample_base + s, U.frame, U.pose_base + (uint32_t)p);
-          ray = generate_ray(o, pose, rnd);
+          const U4 rnd = sample_randoms(U.seed, U.eye, (uint32_t)i, U.sample_base + s, U.frame,
+                                        U.pose_base + (uint32_t)p);
+          ray = generate_ray(U.omms[i], U.poses[p], rnd);
         } else {
           ray.ox = ray.oy = ray.oz = 0.0f;
           ray.dx = ray.dy = 0.0f;
@@ -424,18 +490,18 @@ __global__ __launch_bounds__(kTraceThreads) void k_render_units(SceneView S, Uni
           }
         }
       }
-      r = wave_sum(r);
-      gg = wave_sum(gg);
-      b = wave_sum(b);
+      r = segment_sum(r, seg);
+      gg = segment_sum(gg, seg);
+      b = segment_sum(b, seg);
       if (kStats) {
-        sr = wave_sum(sr);
-        sg = wave_sum(sg);
-        sb = wave_sum(sb);
-        sd = wave_sum(sd);
-        sd2 = wave_sum(sd2);
-        sh = wave_sum(sh);
+        sr = segment_sum(sr, seg);
+        sg = segment_sum(sg, seg);
+        sb = segment_sum(sb, seg);
+        sd = segment_sum(sd, seg);
+        sd2 = segment_sum(sd2, seg);
+        sh = segment_sum(sh, seg);
       }
-      if (lane == 0) {
+      if (slot >= 0 && lane == seg_off) {
         U.colours[out] = make_float4(r, gg, b, (float)n);
         if (kStats) {
           // E[c^2] - E[c]^2 with E[c] the unit's mean colour
@@ -447,6 +513,7 @@ __global__ __launch_bounds__(kTraceThreads) void k_render_units(SceneView S, Uni
           U.stats[2 * out + 1] = make_float4(md, fmaxf(sd2 * inv - md * md, 0.0f), 0.0f, 0.0f);
         }
       }
+      g = g_end;
     }
   }
 }

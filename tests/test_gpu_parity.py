@@ -203,6 +203,33 @@ def test_uneven_sample_counts(lib, config1, oracle_c1):
     er.set_sample_counts(lib, None)
 
 
+@pytest.mark.parametrize("grab", [1, 3, 16, 64])
+def test_packed_small_units(lib, config1, oracle_c1, grab):
+    """Units of at most 32 samples share a wave in power-of-two lane segments;
+    every packing (grab sizes, mixed segment sizes, zero counts, a 33-sample
+    unit breaking a pack) gives the oracle's colours and samples exactly."""
+    omm = config1["eyes"][0][:400]
+    rng = np.random.default_rng(grab)
+    counts = rng.integers(0, 34, size=400).astype(np.uint32)
+    counts[:16] = [1, 1, 2, 3, 4, 5, 8, 16, 17, 32, 33, 0, 7, 31, 9, 1]
+    setup_eye(lib, config1, omm=omm, spp=4)
+    er.set_sample_counts(lib, counts)
+    lib.setRenderTuning(grab, 0)
+    try:
+        assert lib.renderFrame() >= 0, er.last_error(lib)
+        ref = oracle_c1.render(omm, er.pose_np(lib), counts=counts, want_samples=True)
+        assert np.array_equal(er.samples_np(lib).view(np.uint32), ref["samples"].view(np.uint32))
+        assert np.array_equal(er.colours_np(lib), ref["colour_gpu_order"][0])
+        poses6 = scenes.batch_poses(config1["scene"], 5, seed=grab, extent=10.0)
+        _, rgb = er.render_batch_np(lib, poses6)
+        poses12 = np.stack([oracle.pose12(p) for p in poses6])
+        want = oracle_c1.render(omm, poses12, counts=counts, frame=lib.getCurrentEyeFrameIndex())
+        assert np.array_equal(rgb, want["colour_gpu_order"])
+    finally:
+        lib.setRenderTuning(4, 0)
+        er.set_sample_counts(lib, None)
+
+
 def test_pose_batch(lib, config1, oracle_c1):
     omm = config1["eyes"][0][:300]
     setup_eye(lib, config1, omm=omm, spp=16)
