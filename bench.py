@@ -300,6 +300,118 @@ def bench_adaptive(lib, er, scenes, cfg, eyes, target: float = 0.01, n_test: int
     }
 
 
+def bench_large(lib, er, scenes, tmp: str, target: float = 0.01, n_ref: int = 100_000) -> dict:
+    """Large heterogeneous batch: 256 eyes (log-uniform 1k-30k ommatidia,
+    Fibonacci-sphere and ellipsoidal layouts, acceptance 1-5 deg) at 256 poses
+    over a 13.4M-triangle scene, rendered as one batch (all eyes' ommatidia
+    moved to world space and traced in one persistent launch), learned
+    adaptive sampling against uniform sampling at the same RMS error."""
+    t0 = time.perf_counter()
+    scene = scenes.config5_scene()
+    eyes = scenes.config5_eyes()
+    scene.eyes = [scenes.Eye("batch", eyes[0])]
+    path = scenes.write_gltf(os.path.join(tmp, "c5"), "config5", scene)
+    gen_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    lib.loadGlTFscene(str(path).encode())
+    if er.last_error(lib):
+        raise RuntimeError(er.last_error(lib))
+    bvh = er.bvh_info(lib)
+    load_s = time.perf_counter() - t0
+    log(f"C5: {scene.tri_count()} triangles generated in {gen_s:.1f} s, loaded + BVH in {load_s:.1f} s")
+    er.gotoFirstCompoundEye(lib)
+
+    def world_batch(seed):
+        poses = scenes.batch_poses(scene, len(eyes), seed=seed, extent=80.0)
+        return np.concatenate([scenes.world_ommatidia(e, scenes.pose_matrix12(p)) for e, p in zip(eyes, poses)])
+
+    identity = np.zeros((1, 6), np.float32)
+    # training on held-out poses
+    t0 = time.perf_counter()
+    train = world_batch(seed=901)
+    er.set_ommatidia_np(lib, train)
+    lib.setCameraPose(0.0, 0.0, 0.0, 0.0, 0.0, 0.0)
+    rng = np.random.default_rng(17)
+    mask = (rng.random((1, len(train))) < 0.25).astype(np.uint8)
+    lib.setCurrentEyeFrameIndex(77)
+    F, L = er.training_set_np(lib, identity, unit_mask=mask, pilot=8, label_spp=1024, error_target=target)
+    if lib.mlpInit(5) != 0:
+        raise RuntimeError(er.last_error(lib))
+    losses = er.mlp_train_np(lib, F, L, epochs=6, batch=4096, lr=2e-3, seed=6)
+    train_s = time.perf_counter() - t0
+    log(f"C5: trained on {len(L)} rows in {train_s:.1f} s")
+    # the test batch; reference colours of its first n_ref ommatidia at 4,096 spp
+    batch = world_batch(seed=902)
+    n_omm = len(batch)
+    er.set_ommatidia_np(lib, batch[:n_ref])
+    lib.setCurrentEyeSamplesPerOmmatidium(4096)
+    lib.setCurrentEyeFrameIndex(1 << 20)
+    ref = er.render_batch_np(lib, identity)[1][0]
+    er.set_ommatidia_np(lib, batch)
+    lib.setCameraPose(0.0, 0.0, 0.0, 0.0, 0.0, 0.0)
+
+    def uniform(spp, want_rgb=True):
+        lib.setCurrentEyeSamplesPerOmmatidium(spp)
+        lib.setCurrentEyeFrameIndex(0)
+        return er.render_batch_np(lib, identity, want_rgb=want_rgb)
+
+    def err_of(rgb):
+        return rms_abs(rgb[0, :n_ref], ref)
+
+    e64 = err_of(uniform(64)[1])
+    spp_u = max(1, int(math.ceil(64 * (e64 / target) ** 2)))
+    e_u = err_of(uniform(spp_u)[1])
+    while e_u > target and spp_u < 4096:
+        spp_u = int(math.ceil(spp_u * 1.1))
+        e_u = err_of(uniform(spp_u)[1])
+
+    def adaptive(scale, want_rgb=True):
+        lib.setCurrentEyeFrameIndex(0)
+        return er.render_adaptive_np(lib, identity, error_target=target, max_spp=512, budget_scale=scale,
+                                     want_rgb=want_rgb)
+
+    scale = 1.0
+    _, rgb, _, total = adaptive(scale)
+    e_a = err_of(rgb)
+    for _ in range(4):
+        if e_a <= target:
+            break
+        scale *= (e_a / target) ** 2 * 1.05
+        _, rgb, _, total = adaptive(scale)
+        e_a = err_of(rgb)
+
+    def timed(fn, repeats=3):
+        fn()
+        t = time.perf_counter()
+        for _ in range(repeats):
+            fn()
+        return (time.perf_counter() - t) / repeats
+
+    t_a = timed(lambda: adaptive(scale, want_rgb=False))
+    t_u = timed(lambda: uniform(spp_u, want_rgb=False))
+    frames = len(eyes)
+    return {
+        "workload": f"{len(eyes)} eyes ({n_omm} ommatidia, log-uniform 1k-30k per eye) at {len(eyes)} poses, "
+                    f"{scene.tri_count()} triangles (2048^2 heightfield + 20,000 cylinders + 200 rocks), "
+                    "one batch per render",
+        "eye_frames_per_s_adaptive": frames / t_a,
+        "eye_frames_per_s_uniform_batched": frames / t_u,
+        "gain_vs_uniform_batched": t_u / t_a,
+        "grays_per_s_adaptive": total / t_a / 1e9,
+        "grays_per_s_uniform": n_omm * spp_u / t_u / 1e9,
+        "uniform_spp": spp_u,
+        "adaptive_mean_spp": total / n_omm,
+        "rms_abs_uniform": e_u,
+        "rms_abs_adaptive": e_a,
+        "rms_reference": f"first {n_ref} ommatidia of the batch, uniform 4,096-spp render on the GPU",
+        "budget_scale": scale,
+        "bvh": bvh,
+        "scene_load_s": load_s,
+        "training": {"rows": int(len(L)), "label_spp": 1024, "seconds": train_s, "loss_first": float(losses[0]),
+                     "loss_last": float(losses[-1])},
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -310,6 +422,8 @@ def main() -> int:
                     help="render loop only (no 4096-spp error check, no CPU baseline): for profiler runs")
     ap.add_argument("--skip-batched", action="store_true", help="skip the 4,096-pose batched configuration")
     ap.add_argument("--skip-adaptive", action="store_true", help="skip the learned adaptive sampling comparison")
+    ap.add_argument("--large", action="store_true",
+                    help="also run the 256-eye, 13.4M-triangle heterogeneous batch (several minutes)")
     ap.add_argument("--trace-mode", type=int, default=1, choices=(0, 1, 2),
                     help="traversal: 0 per-ray, 1 wave packets (default), 2 wave packets with scalar node loads")
     ap.add_argument("--copy-probe", type=float, default=0.0,
@@ -439,6 +553,9 @@ def main() -> int:
     if not args.profile and not args.skip_adaptive:
         log("learned adaptive sampling ...")
         out["learned_adaptive"] = bench_adaptive(lib, er, scenes, cfg, eyes)
+    if args.large:
+        log("large heterogeneous batch ...")
+        out["large_batch"] = bench_large(lib, er, scenes, tmp)
     if not args.profile and not args.skip_batched:
         log("batched configuration ...")
         out["batched"] = bench_batched(lib, er, scenes, tmp)

@@ -297,6 +297,68 @@ def cylinders(count: int, scene_hf, rng: np.random.Generator, extent: float, seg
 
 # ------------------------------------------------------------------- eyes ----
 
+def icosphere(subdivisions: int):
+    """Unit icosphere: (vertices (V, 3) float64, faces (F, 3) int64), F = 20 * 4^s."""
+    t = (1.0 + math.sqrt(5.0)) / 2.0
+    v = [(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t), (0, -1, -t), (0, 1, -t),
+         (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)]
+    f = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4), (11, 10, 2), (10, 7, 6),
+         (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8), (3, 8, 9), (4, 9, 5), (2, 4, 11), (6, 2, 10),
+         (8, 6, 7), (9, 8, 1)]
+    verts = [np.array(p, float) / np.linalg.norm(p) for p in v]
+    faces = np.array(f, np.int64)
+    for _ in range(subdivisions):
+        cache = {}
+
+        def mid(a, b):
+            key = (a, b) if a < b else (b, a)
+            if key not in cache:
+                m = verts[a] + verts[b]
+                verts.append(m / np.linalg.norm(m))
+                cache[key] = len(verts) - 1
+            return cache[key]
+
+        nf = []
+        for a, b, c in faces:
+            ab, bc, ca = mid(a, b), mid(b, c), mid(c, a)
+            nf += [(a, ab, ca), (b, bc, ab), (c, ca, bc), (ab, bc, ca)]
+        faces = np.array(nf, np.int64)
+    return np.array(verts), faces
+
+
+def rocks(count: int, scene_hf, rng: np.random.Generator, extent: float, subdivisions: int = 5) -> Mesh:
+    """`count` noisy icosphere rocks (radius 0.1-0.6 m) half sunk into the terrain."""
+    xs, zs, H = scene_hf
+    sv, sf = icosphere(subdivisions)
+    verts, idxs, cols = [], [], []
+    for k in range(count):
+        cx, cz = rng.uniform(-extent / 2 * 0.9, extent / 2 * 0.9, size=2)
+        r = rng.uniform(0.1, 0.6)
+        gi = int(np.clip(np.searchsorted(zs, cz), 0, len(zs) - 1))
+        gj = int(np.clip(np.searchsorted(xs, cx), 0, len(xs) - 1))
+        bump = 1.0 + 0.25 * value_noise(sv[:, 0] * 3 + k, sv[:, 2] * 3 + sv[:, 1], rng, 3, 1.0)
+        v = sv * (r * bump)[:, None] * np.array([1.0, rng.uniform(0.5, 0.9), 1.0]) + np.array([cx, H[gi, gj], cz])
+        grey = rng.uniform(0.3, 0.6)
+        c = np.clip(grey + rng.normal(0.0, 0.05, size=(v.shape[0], 3)), 0.0, 1.0)
+        verts.append(v)
+        idxs.append(sf + sum(len(x) for x in verts[:-1]))
+        cols.append(np.concatenate([c, np.ones((v.shape[0], 1))], axis=1))
+    return Mesh("rocks", np.concatenate(verts).astype(np.float32), np.concatenate(idxs).astype(np.uint32),
+                np.concatenate(cols).astype(np.float32))
+
+
+def world_ommatidia(omm: np.ndarray, pose12: np.ndarray) -> np.ndarray:
+    """Ommatidia moved into world space by a pose (position, x, y, z axes): an
+    eye rendered at the identity pose with these ommatidia sees what the
+    original eye sees at `pose` (up to rounding)."""
+    P = np.asarray(pose12, np.float64)
+    R = np.stack([P[3:6], P[6:9], P[9:12]], axis=1)
+    out = omm.astype(np.float64).copy()
+    out[:, 0:3] = omm[:, 0:3].astype(np.float64) @ R.T + P[0:3]
+    out[:, 3:6] = omm[:, 3:6].astype(np.float64) @ R.T
+    return out.astype(np.float32)
+
+
 def fibonacci_eye(n: int, radius: float, acceptance: float) -> np.ndarray:
     """n ommatidia on a Fibonacci sphere, axes radial, focal offset 0."""
     k = np.arange(n, dtype=np.float64)
@@ -520,6 +582,33 @@ def config2_pose(scene: Scene, seed: int = 2) -> np.ndarray:
     rng = np.random.default_rng(seed + 1000)
     yaw = float(rng.uniform(0.0, 2 * math.pi))
     return np.array([0.0, scene.ground_height(0.0, 0.0) + 0.3, 0.0, 0.0, yaw, 0.0], dtype=np.float32)
+
+
+def config5_scene(seed: int = 5, grid: int = 2048, extent: float = 80.0, n_cyl: int = 20000,
+                  n_rocks: int = 200) -> Scene:
+    rng = np.random.default_rng(seed)
+    terrain, hf = heightfield(grid, extent, 3.0, 7, 0.04, rng, colour="earth")
+    veg = cylinders(n_cyl, hf, rng, extent)
+    stones = rocks(n_rocks, hf, rng, extent)
+    return Scene([terrain, veg, stones], heightfield=hf)
+
+
+def config5_eyes(seed: int = 5, count: int = 256, lo: int = 1000, hi: int = 30000):
+    """`count` eyes, ommatidium counts log-uniform in [lo, hi], alternating
+    Fibonacci-sphere and ellipsoidal-patch layouts, acceptance U[1, 5] deg."""
+    rng = np.random.default_rng(seed + 500)
+    sizes = log_uniform_counts(count, lo, hi, seed + 501)
+    eyes = []
+    for k, n in enumerate(sizes):
+        if k % 2 == 0:
+            e = fibonacci_eye(int(n), 0.001, 1.0 * DEG)
+            e[:, 6] = rng.uniform(1.0 * DEG, 5.0 * DEG, size=int(n)).astype(np.float32)
+        else:
+            axes = tuple(rng.uniform(0.0005, 0.0012, size=3))
+            e = ellipsoid_patch_eye(int(n), axes, rng.uniform(90, 200) * DEG, rng.uniform(60, 150) * DEG,
+                                    rng.uniform(-60, 60) * DEG, (1.0 * DEG, 5.0 * DEG), rng)
+        eyes.append(e)
+    return eyes
 
 
 def batch_poses(scene: Scene, count: int, seed: int, extent: float, height=(0.05, 1.0),
