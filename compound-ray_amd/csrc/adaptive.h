@@ -55,6 +55,9 @@ class AdaptiveRenderer {
   bool training_set(const DeviceScene& scene, UnitLaunch U, const std::vector<Pose>& poses,
                     const std::vector<uint8_t>& unit_mask, int pilot, int label_spp, float error_target,
                     std::vector<float>* features, std::vector<float>* labels, hipStream_t stream, std::string* err);
+  // Pilot and top-up colours (r, g, b, samples) and top-up counts of the
+  // first `units` units of the last render.
+  bool copy_last(size_t units, float4* pilot, float4* topup, uint32_t* counts, std::string* err);
   void release();
 
  private:
@@ -63,6 +66,7 @@ class AdaptiveRenderer {
   size_t poses_cap_ = 0;
   float4* d_col_a_ = nullptr;     // pilot colours
   float4* d_col_b_ = nullptr;     // top-up colours
+  float4* d_final_ = nullptr;     // combined colours
   float4* d_stats_ = nullptr;     // pilot statistics (2 per unit)
   float* d_feat_ = nullptr;       // 12 per unit
   float* d_pred_ = nullptr;       // per unit

@@ -74,22 +74,23 @@ __global__ __launch_bounds__(kThreads) void k_allocate(const float* __restrict__
   atomicAdd(total, mine);
 }
 
-__global__ void k_combine(const float4* __restrict__ a, float4* __restrict__ b, const uint32_t* __restrict__ extra,
-                          int pilot, uint64_t units) {
+// Final colour of each unit into `out`: the pilot mean alone, or pilot and
+// top-up means weighted by their sample counts (the top-up buffer is kept).
+__global__ void k_combine(const float4* __restrict__ a, const float4* __restrict__ b,
+                          const uint32_t* __restrict__ extra, int pilot, uint64_t units, float4* __restrict__ out) {
   const uint64_t u = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (u >= units) return;
   const float4 ca = a[u];
   const uint32_t e = extra[u];
   if (e == 0) {
-    b[u] = make_float4(ca.x, ca.y, ca.z, (float)pilot);
+    out[u] = make_float4(ca.x, ca.y, ca.z, (float)pilot);
     return;
   }
   const float4 cb = b[u];
   const float wa = (float)pilot, wb = (float)e, inv = 1.0f / (wa + wb);
-  b[u] = make_float4((wa * ca.x + wb * cb.x) * inv, (wa * ca.y + wb * cb.y) * inv, (wa * ca.z + wb * cb.z) * inv,
-                     wa + wb);
+  out[u] = make_float4((wa * ca.x + wb * cb.x) * inv, (wa * ca.y + wb * cb.y) * inv, (wa * ca.z + wb * cb.z) * inv,
+                       wa + wb);
 }
-
 // Training labels from a long render's statistics: log2(max(n_req, 1)), n_req
 // = mean RGB variance / error^2, clamped to [0, 12].
 __global__ void k_labels(const float4* __restrict__ stats, uint64_t units, float inv_err2, float* __restrict__ lab) {
@@ -128,11 +129,11 @@ bool ok_launch(const char* what, std::string* err) {
 }  // namespace
 
 void AdaptiveRenderer::release() {
-  for (void* p : {(void*)d_poses_, (void*)d_col_a_, (void*)d_col_b_, (void*)d_stats_, (void*)d_feat_,
+  for (void* p : {(void*)d_poses_, (void*)d_col_a_, (void*)d_col_b_, (void*)d_final_, (void*)d_stats_, (void*)d_feat_,
                   (void*)d_pred_, (void*)d_counts_, (void*)d_extra_, (void*)d_total_, (void*)d_queue_})
     if (p) (void)hipFree(p);
   d_poses_ = nullptr;
-  d_col_a_ = d_col_b_ = d_stats_ = nullptr;
+  d_col_a_ = d_col_b_ = d_final_ = d_stats_ = nullptr;
   d_feat_ = d_pred_ = nullptr;
   d_counts_ = d_extra_ = d_queue_ = nullptr;
   d_total_ = nullptr;
@@ -153,7 +154,8 @@ bool AdaptiveRenderer::ensure(size_t poses, size_t units, std::string* err) {
     poses_cap_ = poses;
   }
   if (units > units_cap_) {
-    if (!grow(&d_col_a_, units, err) || !grow(&d_col_b_, units, err) || !grow(&d_stats_, 2 * units, err) ||
+    if (!grow(&d_col_a_, units, err) || !grow(&d_col_b_, units, err) || !grow(&d_final_, units, err) ||
+        !grow(&d_stats_, 2 * units, err) ||
         !grow(&d_feat_, units * kMlpIn, err) || !grow(&d_pred_, units, err) || !grow(&d_counts_, units, err) ||
         !grow(&d_extra_, units, err))
       return false;
@@ -213,7 +215,7 @@ bool AdaptiveRenderer::render(const DeviceScene& scene, UnitLaunch U, const std:
   B.stats = nullptr;
   if (!launch_render_units(scene, B, stream, err)) return false;
   hipLaunchKernelGGL(k_combine, dim3(gb), dim3(kThreads), 0, stream, d_col_a_, d_col_b_, d_extra_, cfg.pilot,
-                     (uint64_t)units);
+                     (uint64_t)units, d_final_);
   if (!ok_launch("combine", err)) return false;
   (void)hipEventRecord(ev1_, stream);
   unsigned long long total = 0;
@@ -230,7 +232,7 @@ bool AdaptiveRenderer::render(const DeviceScene& scene, UnitLaunch U, const std:
   }
   if (rgb) {
     std::vector<float4> tmp(units);
-    if (hipMemcpy(tmp.data(), d_col_b_, sizeof(float4) * units, hipMemcpyDeviceToHost) != hipSuccess) {
+    if (hipMemcpy(tmp.data(), d_final_, sizeof(float4) * units, hipMemcpyDeviceToHost) != hipSuccess) {
       if (err) *err = "adaptive: readback failed";
       return false;
     }
@@ -309,6 +311,20 @@ bool AdaptiveRenderer::training_set(const DeviceScene& scene, UnitLaunch U, cons
     if (!unit_mask[u]) continue;
     features->insert(features->end(), f.begin() + (ptrdiff_t)(u * kMlpIn), f.begin() + (ptrdiff_t)((u + 1) * kMlpIn));
     labels->push_back(l[u]);
+  }
+  return true;
+}
+
+bool AdaptiveRenderer::copy_last(size_t units, float4* pilot, float4* topup, uint32_t* counts, std::string* err) {
+  if (units > units_cap_) {
+    if (err) *err = "adaptive: fewer units were rendered";
+    return false;
+  }
+  if ((pilot && hipMemcpy(pilot, d_col_a_, sizeof(float4) * units, hipMemcpyDeviceToHost) != hipSuccess) ||
+      (topup && hipMemcpy(topup, d_col_b_, sizeof(float4) * units, hipMemcpyDeviceToHost) != hipSuccess) ||
+      (counts && hipMemcpy(counts, d_extra_, sizeof(uint32_t) * units, hipMemcpyDeviceToHost) != hipSuccess)) {
+    if (err) *err = "adaptive: readback failed";
+    return false;
   }
   return true;
 }

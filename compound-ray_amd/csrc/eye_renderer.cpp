@@ -1092,6 +1092,16 @@ double renderPoseBatch(const float* poses, size_t poseCount, float* rgbOut) {
 
 void setBatchAutotune(int enabled) { st().autotune = enabled != 0; }
 
+int64_t getAdaptiveStages(float* pilot, float* topup, uint32_t* extra, size_t maxUnits) {
+  ABI_GUARD_BEGIN
+  std::string err;
+  if (!st().adaptive.copy_last(maxUnits, reinterpret_cast<float4*>(pilot), reinterpret_cast<float4*>(topup), extra,
+                               &err))
+    throw std::runtime_error(err);
+  return (int64_t)maxUnits;
+  ABI_GUARD_END(return -1)
+}
+
 int setAdaptiveLabelError(float labelError) {
   if (!(labelError > 0.0f)) {
     set_error("setAdaptiveLabelError: the error must be positive");

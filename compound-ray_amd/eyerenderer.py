@@ -132,6 +132,8 @@ def configure_extensions(lib):
     lib.renderPoseBatchAdaptive.argtypes = [c_void_p, c_size_t, c_float, c_int, c_float, c_void_p, c_void_p,
                                             c_void_p]
     lib.renderPoseBatchAdaptive.restype = c_double
+    lib.getAdaptiveStages.argtypes = [c_void_p, c_void_p, c_void_p, c_size_t]
+    lib.getAdaptiveStages.restype = c_int64
     lib.setAdaptiveLabelError.argtypes = [c_float]
     lib.setAdaptiveLabelError.restype = c_int
     lib.setTraversalMode.argtypes = [c_int]
@@ -382,6 +384,17 @@ def render_adaptive_np(lib, poses6, error_target=0.01, max_spp=512, budget_scale
     if ms < 0:
         raise RuntimeError(last_error(lib))
     return ms, rgb, counts, int(total[0])
+
+
+def adaptive_stages_np(lib, units: int):
+    """(pilot (units, 4), top-up (units, 4), top-up counts) of the last adaptive render."""
+    a = np.zeros((units, 4), np.float32)
+    b = np.zeros((units, 4), np.float32)
+    e = np.zeros(units, np.uint32)
+    if lib.getAdaptiveStages(a.ctypes.data_as(c_void_p), b.ctypes.data_as(c_void_p), e.ctypes.data_as(c_void_p),
+                             units) < 0:
+        raise RuntimeError(last_error(lib))
+    return a, b, e
 
 
 def bvh_info(lib) -> dict:

@@ -189,6 +189,11 @@ int64_t adaptiveTrainingSet(const float* poses, size_t poseCount, const uint8_t*
  * allocations scale by (label error / errorTarget)^2.
  * rgbOut / countsOut (nullable) per (pose, ommatidium); *totalSamples gets the
  * samples traced.  Returns GPU milliseconds, or -1. */
+/* The two stages of the last renderPoseBatchAdaptive, per (pose, ommatidium)
+ * unit: pilot and top-up colours as (r, g, b, samples) and the top-up sample
+ * counts (any pointer may be NULL).  Returns maxUnits, or -1 when the last
+ * render had fewer units. */
+int64_t getAdaptiveStages(float* pilot, float* topup, uint32_t* extra, size_t maxUnits);
 /* Error the sampler network's labels were made for (default 0.01). */
 int setAdaptiveLabelError(float labelError);
 double renderPoseBatchAdaptive(const float* poses, size_t poseCount, float errorTarget, int maxSpp,

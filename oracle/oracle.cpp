@@ -1062,7 +1062,8 @@ void oracle_radiance(void* h, const float* rays, int64_t n, float* rgb, int thre
   parallel_for((size_t)n, threads, [&](size_t k) {
     RayO r;
     std::memcpy(&r, rays + 8 * k, sizeof(RayO));
-    const HitO hh = trace_bvh(S, r, nullptr, nullptr);
+    uint64_t nv = 0, nt = 0;
+    const HitO hh = trace_bvh(S, r, &nv, &nt);
     const F3 c = shade(S, hh, r);
     rgb[3 * k] = c.x;
     rgb[3 * k + 1] = c.y;

@@ -506,3 +506,27 @@ def test_camera_rays_closed_forms():
     assert np.allclose(pano[2, 6, 4:7], [1, 0, 0], atol=1e-6)    # lon = 0
     assert np.allclose(pano[2, 2, 4:7], [-1, 0, 0], atol=1e-6)   # lon = pi
     assert np.all(pano[..., 0:3] == [1, 2, 3])
+
+
+def test_radiance_is_shaded_trace(oracle_c1, config1):
+    """oracle radiance = shading of the closest hit, or the sky on a miss: rays
+    straight down hit the terrain (vertex-colour shading); rays straight up see
+    the top sky colour."""
+    n = 64
+    rng = np.random.default_rng(2)
+    rays = np.zeros((2 * n, 8), np.float32)
+    rays[:, 0] = rng.uniform(-4, 4, 2 * n)
+    rays[:, 2] = rng.uniform(-4, 4, 2 * n)
+    rays[:, 1] = 5.0
+    rays[:n, 5] = -1.0
+    rays[n:, 5] = 1.0
+    rays[:, 3] = 0.0
+    rays[:, 7] = 1e16
+    col = oracle_c1.radiance(rays)
+    ids, _ = oracle_c1.trace(rays)
+    assert np.all(ids[:n] >= 0) and np.all(ids[n:] < 0)
+    upper = np.float32([1, 31, 117]) / np.float32(255)
+    assert np.allclose(col[n:], upper, atol=1e-6)
+    assert np.all(col[:n] >= 0) and np.all(col[:n] <= 1)
+    ref = oracle_c1.render(config1["eyes"][0][:4], oracle.pose12(config1["pose"]), spp=1, want_samples=True)
+    assert ref["samples"].shape[0] == 4
