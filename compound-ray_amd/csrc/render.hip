@@ -449,7 +449,8 @@ __global__ __launch_bounds__(kTraceThreads) void k_render_units(SceneView S, Uni
       int p, i;
       uint64_t out;
       locate_unit(U, (uint64_t)base + (uint64_t)src, &p, &i, &out);
-      const uint32_t n = slot < 0 ? 0u : (uint32_t)__shfl((int)my_n, src);
+      const uint32_t n_src = (uint32_t)__shfl((int)my_n, src);  // every lane takes part in the shuffle
+      const uint32_t n = slot < 0 ? 0u : n_src;
       const float inv = 1.0f / (float)(n > 0u ? n : 1u);
       const uint32_t s_lane = (uint32_t)(lane - seg_off);
       float r = 0.0f, gg = 0.0f, b = 0.0f;
