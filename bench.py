@@ -139,7 +139,7 @@ def bench_batched(lib, er, scenes, tmp: str, repeats: int = 3) -> dict:
         "eye_frames_per_s_gpu": len(poses) / (gpu_ms * 1e-3),
         "autotune": {"shape": tuning, "candidates": int(len(trials)), "seconds": tune_s,
                      "sample_ms_best": float(trials[:, 3].min()), "sample_ms_worst": float(trials[:, 3].max()),
-                     "sample_ms_grab4_vector_loads": default_ms},
+                     "sample_ms_grab4_packet": default_ms},
     }
 
 

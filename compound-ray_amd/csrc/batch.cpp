@@ -31,7 +31,7 @@ bool same_launch(const UnitLaunch& a, const UnitLaunch& b) {
          a.omm_counts == b.omm_counts && a.unit_counts == b.unit_counts && a.sample_base == b.sample_base &&
          a.order == b.order && a.colours == b.colours && a.stats == b.stats && a.samples == b.samples &&
          a.sample_offsets == b.sample_offsets && a.queue == b.queue && a.units_per_grab == b.units_per_grab &&
-         a.blocks == b.blocks && a.scalar_loads == b.scalar_loads;
+         a.blocks == b.blocks && a.traversal == b.traversal;
 }
 }  // namespace
 
@@ -39,13 +39,13 @@ std::vector<LaunchShape> LaunchTuner::candidates(int capacity) {
   std::vector<LaunchShape> out;
   const int cap = std::max(capacity, 1);
   const int block_opts[3] = {std::max(cap / 2, 1), cap, 2 * cap};
-  for (int scalar = 0; scalar < 2; ++scalar)
+  for (int trav = kUnitPacket; trav <= kUnitPerRay; ++trav)
     for (int b : block_opts)
       for (int g : {1, 2, 4, 8, 16}) {
         LaunchShape sh;
         sh.units_per_grab = g;
         sh.blocks = b;
-        sh.scalar_loads = scalar;
+        sh.traversal = trav;
         out.push_back(sh);
       }
   return out;
@@ -92,8 +92,8 @@ bool LaunchTuner::load(const std::string& path, std::string* err) {
     std::string key;
     LaunchShape sh;
     double ms = 0.0;
-    if (!(in >> key >> sh.units_per_grab >> sh.blocks >> sh.scalar_loads >> ms) || sh.units_per_grab < 1 ||
-        sh.blocks < 0 || (sh.scalar_loads != 0 && sh.scalar_loads != 1)) {
+    if (!(in >> key >> sh.units_per_grab >> sh.blocks >> sh.traversal >> ms) || sh.units_per_grab < 1 ||
+        sh.blocks < 0 || sh.traversal < kUnitPacket || sh.traversal > kUnitPerRay) {
       if (err) *err = "autotune cache " + path + ": bad entry on line " + std::to_string(lineno);
       return false;
     }
@@ -108,9 +108,9 @@ bool LaunchTuner::save(const std::string& path, std::string* err) const {
     if (err) *err = "autotune cache " + path + ": cannot write";
     return false;
   }
-  f << "# key units_per_grab blocks scalar_loads ms\n";
+  f << "# key units_per_grab blocks traversal ms\n";
   for (size_t i = 0; i < keys_.size(); ++i)
-    f << keys_[i] << ' ' << shapes_[i].units_per_grab << ' ' << shapes_[i].blocks << ' ' << shapes_[i].scalar_loads
+    f << keys_[i] << ' ' << shapes_[i].units_per_grab << ' ' << shapes_[i].blocks << ' ' << shapes_[i].traversal
       << ' ' << ms_[i] << '\n';
   return bool(f);
 }
@@ -208,7 +208,7 @@ bool BatchRenderer::tune(const DeviceScene& scene, UnitLaunch U, const std::vect
   for (const LaunchShape& sh : LaunchTuner::candidates(capacity)) {
     U.units_per_grab = sh.units_per_grab;
     U.blocks = sh.blocks;
-    U.scalar_loads = sh.scalar_loads;
+    U.traversal = sh.traversal;
     if (!launch_render_units(scene, U, stream, err)) return false;  // untimed: caches, clocks
     double ms_min = 1e300;
     for (int r = 0; r < std::max(repeats, 1); ++r) {

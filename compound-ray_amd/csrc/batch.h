@@ -21,7 +21,7 @@ namespace eye {
 struct LaunchShape {
   int units_per_grab = 4;
   int blocks = 0;  // 0: resident capacity
-  int scalar_loads = 0;
+  int traversal = 0;  // UnitTraversal
 };
 
 struct TuneTrial {
@@ -32,10 +32,11 @@ struct TuneTrial {
 // Launch-shape autotuner: candidate shapes are timed on a sample of a batch and
 // the fastest is kept per workload key (scene, eye and sample-count totals).
 // Entries can be saved to and loaded from a text file, one per line:
-//   <key> <units_per_grab> <blocks> <scalar_loads> <ms>
+//   <key> <units_per_grab> <blocks> <traversal> <ms>
 class LaunchTuner {
  public:
-  // grabs {1,2,4,8,16} x blocks {capacity/2, capacity, 2 capacity} x {vector, scalar} loads
+  // grabs {1,2,4,8,16} x blocks {capacity/2, capacity, 2 capacity} x traversal {packet, packet with
+  // scalar loads, per-ray}; capacity is that of the packet kernel
   static std::vector<LaunchShape> candidates(int capacity);
   static std::string key_of(const DeviceScene& scene, const UnitLaunch& U, uint64_t samples_per_pose);
   bool find(const std::string& key, LaunchShape* shape) const;

@@ -288,7 +288,7 @@ UnitLaunch unit_launch_of(EyeState& e, const Pose* d_poses, int pose_count) {
   U.queue = s.d_queue;
   U.units_per_grab = s.units_per_grab;
   U.blocks = s.unit_blocks;
-  U.scalar_loads = trace_mode() == kTracePacketScalar ? 1 : 0;
+  U.traversal = unit_traversal_of(trace_mode());
   U.order = e.d_order;
   return U;
 }
@@ -321,8 +321,7 @@ double render_compound(EyeState& e, const Cam& c) {
   const uint64_t total = e.total_samples();
   // per-sample records are kept only when asked for (or for the raw projection)
   const bool record = s.record_samples || e.shader == "raw_ommatidial_samples";
-  const bool units = trace_mode() != kTracePerRay;
-  if (record || !units) ensure_device(&e.d_samples, &e.d_samples_cap, (size_t)total);
+  if (record) ensure_device(&e.d_samples, &e.d_samples_cap, (size_t)total);
   ensure_device(&e.d_colours, &e.d_colours_cap, e.omms.size());
   ensure_frame_buffer();
   hip_check(hipMemcpyAsync(s.d_pose, &c.pose, sizeof(Pose), hipMemcpyHostToDevice, s.stream), "upload pose");
@@ -330,19 +329,13 @@ double render_compound(EyeState& e, const Cam& c) {
   std::string err;
   const auto t0 = std::chrono::steady_clock::now();
   hip_check(hipEventRecord(s.ev[0], s.stream), "event");
-  if (units) {
-    UnitLaunch U = unit_launch_of(e, s.d_pose, 1);
-    if (record) {
-      U.samples = e.d_samples;
-      U.sample_offsets = e.d_sample_offsets;
-    }
-    if (!launch_render_units(s.scene, U, s.stream, &err)) throw std::runtime_error(err);
-    hip_check(hipEventRecord(s.ev[1], s.stream), "event");
-  } else {
-    if (!launch_trace_samples(s.scene, L, e.d_samples, s.stream, &err)) throw std::runtime_error(err);
-    hip_check(hipEventRecord(s.ev[1], s.stream), "event");
-    if (!launch_reduce(L, e.d_samples, e.d_colours, s.stream, &err)) throw std::runtime_error(err);
+  UnitLaunch U = unit_launch_of(e, s.d_pose, 1);
+  if (record) {
+    U.samples = e.d_samples;
+    U.sample_offsets = e.d_sample_offsets;
   }
+  if (!launch_render_units(s.scene, U, s.stream, &err)) throw std::runtime_error(err);
+  hip_check(hipEventRecord(s.ev[1], s.stream), "event");
   hip_check(hipEventRecord(s.ev[2], s.stream), "event");
   bool ok = true;
   int sph = -1;
@@ -383,7 +376,7 @@ double render_compound(EyeState& e, const Cam& c) {
   s.timings[3] = (double)f[0] + f[1] + f[2];
   e.frame += 1;
   e.has_render = true;
-  e.rendered_rays = (record || !units) ? total : 0;
+  e.rendered_rays = record ? total : 0;
   s.frame_host_stale = true;
   return ms;
 }
@@ -1056,7 +1049,7 @@ static LaunchShape tune_shape(EyeState& e, UnitLaunch U, const std::vector<Pose>
   double best_ms = 0.0;
   for (const TuneTrial& t : *trials)
     if (t.shape.units_per_grab == best.units_per_grab && t.shape.blocks == best.blocks &&
-        t.shape.scalar_loads == best.scalar_loads)
+        t.shape.traversal == best.traversal)
       best_ms = t.ms;
   s.tuner.put(LaunchTuner::key_of(s.scene, U, e.total_samples()), best, best_ms);
   if (!s.tune_cache.empty() && !s.tuner.save(s.tune_cache, &err)) throw std::runtime_error(err);
@@ -1081,7 +1074,7 @@ double renderPoseBatch(const float* poses, size_t poseCount, float* rgbOut) {
     }
     U.units_per_grab = shape.units_per_grab;
     U.blocks = shape.blocks;
-    U.scalar_loads = shape.scalar_loads;
+    U.traversal = shape.traversal;
   }
   std::string err;
   double ms = -1.0;
@@ -1135,12 +1128,12 @@ int autotuneRenderBatch(const float* poses, size_t poseCount, int repeats, float
   // the winner becomes the shape of every later unit launch
   s.units_per_grab = best.units_per_grab;
   s.unit_blocks = best.blocks;
-  set_trace_mode(best.scalar_loads ? kTracePacketScalar : kTracePacket);
+  set_trace_mode(trace_mode_of(best.traversal));
   if (trials)
     for (size_t i = 0; i < tr.size() && i < maxTrials; ++i) {
       trials[4 * i + 0] = (float)tr[i].shape.units_per_grab;
       trials[4 * i + 1] = (float)tr[i].shape.blocks;
-      trials[4 * i + 2] = (float)tr[i].shape.scalar_loads;
+      trials[4 * i + 2] = (float)tr[i].shape.traversal;
       trials[4 * i + 3] = (float)tr[i].ms;
     }
   return (int)tr.size();

@@ -52,8 +52,13 @@ struct UnitLaunch {
   // launch shape (results do not depend on it)
   int units_per_grab = 4;               // consecutive units a wave takes from the queue
   int blocks = 0;                       // blocks of 256 threads; 0: every resident slot once
-  int scalar_loads = 0;                 // read nodes/triangles through the scalar unit
+  int traversal = 0;                    // UnitTraversal
 };
+
+// Traversal inside the unit renderer: the wave walks the BVH as one packet
+// (a wave-uniform node, node records read with vector loads or through the
+// scalar unit), or every lane walks its own ray with an LDS stack per lane.
+enum UnitTraversal : int { kUnitPacket = 0, kUnitPacketScalar = 1, kUnitPerRay = 2 };
 
 // Launches the unit renderer with U's launch shape.  The queue counter is
 // cleared by a kernel first.
@@ -63,17 +68,15 @@ bool launch_render_units(const DeviceScene& scene, const UnitLaunch& U, hipStrea
 // blocks per CU for this scene's stack size and U's kernel variant).
 int render_units_capacity(const DeviceScene& scene, const UnitLaunch& U);
 
-// Traversal of eye rays: one ray per lane with its own LDS stack, or the
-// wave-packet traversal (coherent rays of one ommatidium share the walk),
-// whose node and triangle records are read with vector loads or, in
-// kTracePacketScalar, through the scalar unit (the address is wave-uniform).
+// Traversal of eye rays in renderFrame and the batch paths: 0 = one ray per
+// lane, 1 = wave packets, 2 = wave packets reading nodes and triangles
+// through the scalar unit (UnitTraversal kUnitPerRay / kUnitPacket /
+// kUnitPacketScalar).
 enum TraceMode : int { kTracePerRay = 0, kTracePacket = 1, kTracePacketScalar = 2 };
 void set_trace_mode(int mode);
 int trace_mode();
-
-// Per-sample radiance: samples[ray] = (r, g, b, t) with t < 0 for a miss.
-bool launch_trace_samples(const DeviceScene& scene, const EyeLaunch& L, float4* samples,
-                          hipStream_t stream, std::string* err);
+int unit_traversal_of(int trace_mode);
+int trace_mode_of(int unit_traversal);
 
 // Plain float4 copy of n elements (bandwidth reference for the profiler).
 bool launch_copy_probe(const float4* src, float4* dst, size_t n, hipStream_t stream, std::string* err);
@@ -82,14 +85,6 @@ bool launch_copy_probe(const float4* src, float4* dst, size_t n, hipStream_t str
 // triangles tested, totals[2] += rays that hit (device counters, zeroed by the caller).
 bool launch_trace_stats(const DeviceScene& scene, const EyeLaunch& L, unsigned long long* totals,
                         hipStream_t stream, std::string* err);
-
-// Per-ommatidium averages in the canonical order (lane l of a wave sums
-// samples l, l+64, ... sequentially, then an xor butterfly over the wave):
-// colours[p * omm_count + i] = (r, g, b, n).  `stats`, when not null, gets
-// (depth mean, depth variance, rgb variance mean, hit fraction) per
-// ommatidium over the first `stat_samples` samples.
-bool launch_reduce(const EyeLaunch& L, const float4* samples, float4* colours, hipStream_t stream,
-                   std::string* err);
 
 // uchar4 frame of the single_dimension_fast projection (shaders.cu:396-406):
 // pixel x of row 0 gets make_color(colour[x]) for x < min(width, omm_count).

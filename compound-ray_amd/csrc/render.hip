@@ -208,24 +208,6 @@ __device__ __forceinline__ void locate_ray(const EyeLaunch& L, uint64_t r, int* 
   *n = (int)(L.offsets[lo + 1] - L.offsets[lo]);
 }
 
-__global__ __launch_bounds__(kTraceThreads) void k_trace_samples(SceneView S, EyeLaunch L, uint64_t total,
-                                                                 float4* __restrict__ samples) {
-  extern __shared__ uint32_t lds_stack[];
-  const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= total) return;
-  int p, i, s, n;
-  locate_ray(L, r, &p, &i, &s, &n);
-  const Ommatidium o = L.omms[i];
-  const Pose pose = L.poses[p];
-  const U4 rnd = sample_randoms(L.seed, L.eye, (uint32_t)i, (uint32_t)s, L.frame, L.pose_base + (uint32_t)p);
-  const Ray ray = generate_ray(o, pose, rnd);
-  const RayPre rp = precompute_ray(ray);
-  Hit h;
-  closest_hit(S, rp, lds_stack + threadIdx.x, blockDim.x, h);
-  const V3 c = shade(S, h, ray.dx, ray.dy, ray.dz);
-  samples[r] = make_float4(c.x, c.y, c.z, h.id == 0xffffffffu ? -1.0f : h.t);
-}
-
 // ---------------------------------------------------------------------------
 // Wave-packet traversal.  The 64 rays of a wave are samples of one or two
 // neighbouring ommatidia: nearly the same origin and direction.  The wave
@@ -328,32 +310,6 @@ __device__ __forceinline__ void closest_hit_packet(const SceneView& S, const Ray
     closest_hit_packet_impl<false, kScalar>(S, rp, active, wstack, h);
 }
 
-__global__ __launch_bounds__(kTraceThreads) void k_trace_samples_packet(SceneView S, EyeLaunch L, uint64_t total,
-                                                                        float4* __restrict__ samples, int wstride) {
-  extern __shared__ int wave_stacks[];
-  const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const bool active = r < total;
-  int p = 0, i = 0, s = 0, n = 1;
-  if (active) locate_ray(L, r, &p, &i, &s, &n);
-  Ray ray;
-  if (active) {
-    const U4 rnd = sample_randoms(L.seed, L.eye, (uint32_t)i, (uint32_t)s, L.frame, L.pose_base + (uint32_t)p);
-    ray = generate_ray(L.omms[i], L.poses[p], rnd);
-  } else {
-    ray.ox = ray.oy = ray.oz = 0.0f;
-    ray.dx = ray.dy = 0.0f;
-    ray.dz = 1.0f;
-    ray.tmin = 0.0f;
-    ray.tmax = 1.0f;
-  }
-  const RayPre rp = precompute_ray(ray);
-  Hit h;
-  closest_hit_packet(S, rp, active, wave_stacks + (threadIdx.x >> 6) * wstride, h);
-  if (!active) return;
-  const V3 c = shade(S, h, ray.dx, ray.dy, ray.dz);
-  samples[r] = make_float4(c.x, c.y, c.z, h.id == 0xffffffffu ? -1.0f : h.t);
-}
-
 __device__ __forceinline__ float wave_sum(float v) {
   for (int m = 32; m >= 1; m >>= 1) v = v + __shfl_xor(v, m);
   return v;
@@ -398,11 +354,14 @@ __device__ __forceinline__ int pow2_at_least(uint32_t n) {
 // packed side by side into power-of-two lane segments, so the lanes of a
 // wave stay busy when sample counts are small (adaptive pilots, uneven
 // counts).  Each unit's colour is the same canonical sum either way.
-template <bool kStats, bool kSamples, bool kScalar>
+template <bool kStats, bool kSamples, int kTrav>
 __global__ __launch_bounds__(kTraceThreads) void k_render_units(SceneView S, UnitLaunch U, int wstride) {
+  // LDS: one stack per wave (packets) or one column per lane (per-ray,
+  // laid out [depth][blockDim.x])
   extern __shared__ int wave_stacks[];
   const int lane = threadIdx.x & 63;
   int* wst = wave_stacks + (threadIdx.x >> 6) * wstride;
+  uint32_t* lane_stack = reinterpret_cast<uint32_t*>(wave_stacks) + threadIdx.x;
   const uint64_t units = (uint64_t)U.pose_count * (uint64_t)U.omm_count;
   for (;;) {
     uint32_t base = 0;
@@ -472,7 +431,14 @@ __global__ __launch_bounds__(kTraceThreads) void k_render_units(SceneView S, Uni
         }
         const RayPre rp = precompute_ray(ray);
         Hit h;
-        closest_hit_packet<kScalar>(S, rp, active, wst, h);
+        if (kTrav == kUnitPerRay) {
+          h.t = rp.tmax;
+          h.k = h.id = 0xffffffffu;
+          h.b1 = h.b2 = 0.0f;
+          if (active) closest_hit(S, rp, lane_stack, (int)blockDim.x, h);
+        } else {
+          closest_hit_packet<kTrav == kUnitPacketScalar>(S, rp, active, wst, h);
+        }
         if (active) {
           const V3 c = shade(S, h, ray.dx, ray.dy, ray.dz);
           r = r + c.x * inv;
@@ -575,40 +541,6 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace_rays(SceneView S, const
   tbb[r] = make_float4(h.t, h.b1, h.b2, 0.0f);
 }
 
-// One wave per (pose, ommatidium).
-__global__ __launch_bounds__(256) void k_reduce(EyeLaunch L, const float4* __restrict__ samples,
-                                                float4* __restrict__ colours) {
-  const int lane = threadIdx.x & 63;
-  const uint64_t unit = (uint64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  const uint64_t units = (uint64_t)L.omm_count * (uint64_t)L.pose_count;
-  if (unit >= units) return;
-  const uint64_t p = unit / (uint64_t)L.omm_count;
-  const int i = (int)(unit - p * (uint64_t)L.omm_count);
-  uint32_t first, n;
-  if (L.offsets == nullptr) {
-    first = (uint32_t)i * (uint32_t)L.spp;
-    n = (uint32_t)L.spp;
-  } else {
-    first = L.offsets[i];
-    n = L.offsets[i + 1] - first;
-  }
-  const float inv = 1.0f / (float)(n > 0 ? n : 1u);
-  const float4* base = samples + p * L.rays_per_pose + first;
-  float r = 0.0f, g = 0.0f, b = 0.0f;
-  for (uint32_t s = (uint32_t)lane; s < n; s += 64) {
-    const float4 v = base[s];
-    r = r + v.x * inv;
-    g = g + v.y * inv;
-    b = b + v.z * inv;
-  }
-  for (int m = 32; m >= 1; m >>= 1) {
-    r = r + __shfl_xor(r, m);
-    g = g + __shfl_xor(g, m);
-    b = b + __shfl_xor(b, m);
-  }
-  if (lane == 0) colours[unit] = make_float4(r, g, b, (float)n);
-}
-
 __device__ __forceinline__ uchar4 make_color(float4 c) {
   return make_uchar4(encode_channel(c.x), encode_channel(c.y), encode_channel(c.z), 255);
 }
@@ -681,26 +613,26 @@ size_t stack_bytes(const DeviceScene& s) {
 
 static int g_trace_mode = kTracePacket;
 
-template <bool kScalar>
+template <int kTrav>
 static void launch_units_variant(const UnitLaunch& U, const SceneView& v, int blocks, size_t lds, int wstride,
                                  hipStream_t stream) {
   const dim3 grid(blocks), block(kTraceThreads);
   if (U.stats && U.samples)
-    hipLaunchKernelGGL((k_render_units<true, true, kScalar>), grid, block, lds, stream, v, U, wstride);
+    hipLaunchKernelGGL((k_render_units<true, true, kTrav>), grid, block, lds, stream, v, U, wstride);
   else if (U.stats)
-    hipLaunchKernelGGL((k_render_units<true, false, kScalar>), grid, block, lds, stream, v, U, wstride);
+    hipLaunchKernelGGL((k_render_units<true, false, kTrav>), grid, block, lds, stream, v, U, wstride);
   else if (U.samples)
-    hipLaunchKernelGGL((k_render_units<false, true, kScalar>), grid, block, lds, stream, v, U, wstride);
+    hipLaunchKernelGGL((k_render_units<false, true, kTrav>), grid, block, lds, stream, v, U, wstride);
   else
-    hipLaunchKernelGGL((k_render_units<false, false, kScalar>), grid, block, lds, stream, v, U, wstride);
+    hipLaunchKernelGGL((k_render_units<false, false, kTrav>), grid, block, lds, stream, v, U, wstride);
 }
 
-template <bool kScalar>
+template <int kTrav>
 static const void* units_kernel_variant(const UnitLaunch& U) {
-  if (U.stats && U.samples) return reinterpret_cast<const void*>(&k_render_units<true, true, kScalar>);
-  if (U.stats) return reinterpret_cast<const void*>(&k_render_units<true, false, kScalar>);
-  if (U.samples) return reinterpret_cast<const void*>(&k_render_units<false, true, kScalar>);
-  return reinterpret_cast<const void*>(&k_render_units<false, false, kScalar>);
+  if (U.stats && U.samples) return reinterpret_cast<const void*>(&k_render_units<true, true, kTrav>);
+  if (U.stats) return reinterpret_cast<const void*>(&k_render_units<true, false, kTrav>);
+  if (U.samples) return reinterpret_cast<const void*>(&k_render_units<false, true, kTrav>);
+  return reinterpret_cast<const void*>(&k_render_units<false, false, kTrav>);
 }
 
 static int device_cus() {
@@ -714,15 +646,22 @@ static int device_cus() {
   return cus;
 }
 
-static size_t units_lds_bytes(const DeviceScene& scene) {
+static size_t units_lds_bytes(const DeviceScene& scene, int traversal) {
+  if (traversal == kUnitPerRay) return stack_bytes(scene);
   const int wstride = scene.stack_bound > 0 ? scene.stack_bound : 1;
   return sizeof(int) * (size_t)wstride * (kTraceThreads / 64);
 }
 
+static const void* units_kernel_of(const UnitLaunch& U) {
+  if (U.traversal == kUnitPerRay) return units_kernel_variant<kUnitPerRay>(U);
+  if (U.traversal == kUnitPacketScalar) return units_kernel_variant<kUnitPacketScalar>(U);
+  return units_kernel_variant<kUnitPacket>(U);
+}
+
 int render_units_capacity(const DeviceScene& scene, const UnitLaunch& U) {
-  const void* fn = U.scalar_loads ? units_kernel_variant<true>(U) : units_kernel_variant<false>(U);
+  const void* fn = units_kernel_of(U);
   int per_cu = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kTraceThreads, units_lds_bytes(scene)) !=
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kTraceThreads, units_lds_bytes(scene, U.traversal)) !=
           hipSuccess ||
       per_cu <= 0)
     per_cu = 4;
@@ -740,45 +679,38 @@ bool launch_render_units(const DeviceScene& scene, const UnitLaunch& U, hipStrea
     if (err) *err = "render_units: incomplete launch description";
     return false;
   }
+  if (U.traversal < kUnitPacket || U.traversal > kUnitPerRay) {
+    if (err) *err = "render_units: unknown traversal";
+    return false;
+  }
   const int wstride = scene.stack_bound > 0 ? scene.stack_bound : 1;
-  const size_t lds = units_lds_bytes(scene);
+  const size_t lds = units_lds_bytes(scene, U.traversal);
+  if (lds > 160 * 1024) {
+    if (err) *err = "render_units: BVH too deep for the LDS traversal stack";
+    return false;
+  }
   int blocks = U.blocks > 0 ? U.blocks : render_units_capacity(scene, U);
   const uint64_t need = (units + (uint64_t)U.units_per_grab - 1) / (uint64_t)U.units_per_grab;
   const uint64_t waves = (uint64_t)blocks * (kTraceThreads / 64);
   if (waves > need) blocks = (int)((need + (kTraceThreads / 64) - 1) / (kTraceThreads / 64));
   hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, stream, U.queue, 1u, 0u);
   const SceneView v = view_of(scene);
-  if (U.scalar_loads)
-    launch_units_variant<true>(U, v, blocks, lds, wstride, stream);
+  if (U.traversal == kUnitPerRay)
+    launch_units_variant<kUnitPerRay>(U, v, blocks, lds, wstride, stream);
+  else if (U.traversal == kUnitPacketScalar)
+    launch_units_variant<kUnitPacketScalar>(U, v, blocks, lds, wstride, stream);
   else
-    launch_units_variant<false>(U, v, blocks, lds, wstride, stream);
+    launch_units_variant<kUnitPacket>(U, v, blocks, lds, wstride, stream);
   return check_launch("render_units", err);
 }
 
 void set_trace_mode(int mode) { g_trace_mode = mode; }
 int trace_mode() { return g_trace_mode; }
-
-bool launch_trace_samples(const DeviceScene& scene, const EyeLaunch& L, float4* samples, hipStream_t stream,
-                          std::string* err) {
-  const uint64_t total = L.rays_per_pose * (uint64_t)L.pose_count;
-  if (total == 0) return true;
-  if (g_trace_mode != kTracePerRay) {
-    const int wstride = scene.stack_bound > 0 ? scene.stack_bound : 1;
-    const size_t lds = sizeof(int) * (size_t)wstride * (kTraceThreads / 64);
-    const uint64_t blocks = (total + kTraceThreads - 1) / kTraceThreads;
-    hipLaunchKernelGGL(k_trace_samples_packet, dim3((unsigned)blocks), dim3(kTraceThreads), lds, stream,
-                       view_of(scene), L, total, samples, wstride);
-    return check_launch("trace_samples_packet", err);
-  }
-  const size_t lds = stack_bytes(scene);
-  if (lds > 160 * 1024) {
-    if (err) *err = "trace: BVH too deep for the LDS traversal stack";
-    return false;
-  }
-  const uint64_t blocks = (total + kTraceThreads - 1) / kTraceThreads;
-  hipLaunchKernelGGL(k_trace_samples, dim3((unsigned)blocks), dim3(kTraceThreads), lds, stream, view_of(scene), L,
-                     total, samples);
-  return check_launch("trace_samples", err);
+int unit_traversal_of(int mode) {
+  return mode == kTracePerRay ? kUnitPerRay : mode == kTracePacketScalar ? kUnitPacketScalar : kUnitPacket;
+}
+int trace_mode_of(int traversal) {
+  return traversal == kUnitPerRay ? kTracePerRay : traversal == kUnitPacketScalar ? kTracePacketScalar : kTracePacket;
 }
 
 bool launch_copy_probe(const float4* src, float4* dst, size_t n, hipStream_t stream, std::string* err) {
@@ -801,15 +733,6 @@ bool launch_trace_stats(const DeviceScene& scene, const EyeLaunch& L, unsigned l
   hipLaunchKernelGGL(k_trace_stats, dim3((unsigned)blocks), dim3(kTraceThreads), lds, stream, view_of(scene), L,
                      total, totals);
   return check_launch("trace_stats", err);
-}
-
-bool launch_reduce(const EyeLaunch& L, const float4* samples, float4* colours, hipStream_t stream,
-                   std::string* err) {
-  const uint64_t units = (uint64_t)L.omm_count * (uint64_t)L.pose_count;
-  if (units == 0) return true;
-  const uint64_t blocks = (units + 3) / 4;
-  hipLaunchKernelGGL(k_reduce, dim3((unsigned)blocks), dim3(256), 0, stream, L, samples, colours);
-  return check_launch("reduce", err);
 }
 
 bool launch_frame_single_dim_fast(const float4* colours, int omm_count, uchar4* frame, int width,

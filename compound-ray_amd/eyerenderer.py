@@ -328,9 +328,10 @@ def render_tuning(lib) -> dict:
 
 def autotune_np(lib, poses6: np.ndarray, repeats: int = 1) -> np.ndarray:
     """Times the candidate launch shapes on these poses and installs the
-    fastest; returns rows (units_per_grab, blocks, scalar_loads, ms)."""
+    fastest; returns rows (units_per_grab, blocks, traversal, ms) with
+    traversal 0 = packet, 1 = packet with scalar loads, 2 = per-ray."""
     p = np.ascontiguousarray(poses6, dtype=np.float32).reshape(-1, 6)
-    rows = np.zeros((64, 4), np.float32)
+    rows = np.zeros((128, 4), np.float32)
     n = lib.autotuneRenderBatch(p, p.shape[0], repeats, rows.ctypes.data_as(c_void_p), rows.shape[0])
     if n < 0:
         raise RuntimeError(last_error(lib))

@@ -209,9 +209,9 @@ size_t copySceneSortOrder(uint32_t* ids, uint32_t* codes, size_t count);
 /* Rebuilds the BVH of the loaded scene with another leaf size (1..16). */
 int setSceneLeafSize(int leafSize);
 
-/* Traversal strategy of eye rays: 0 = one ray per lane, 1 = wave packets
- * (default), 2 = wave packets reading nodes and triangles through the scalar
- * unit.  Results are identical; only speed differs. */
+/* Traversal strategy of eye rays: 0 = one ray per lane (own LDS stack),
+ * 1 = wave packets (default), 2 = wave packets reading nodes and triangles
+ * through the scalar unit.  Results are identical; only speed differs. */
 void setTraversalMode(int mode);
 /* Keep the per-sample records of renderFrame for getCurrentEyeSamples (off by
  * default: the packet renderer reduces samples in registers). */
@@ -231,8 +231,8 @@ void setBatchAutotune(int enabled);
 int setAutotuneCacheFile(const char* path);
 /* Times every candidate shape on (a sample of) these poses and installs the
  * fastest as the shape of all later unit launches.  trials receives rows of
- * 4 floats (units per grab, blocks, scalar loads, ms); returns the number of
- * candidates, -1 on error. */
+ * 4 floats (units per grab, blocks, traversal: 0 packet, 1 packet with scalar
+ * loads, 2 per-ray; ms); returns the number of candidates, -1 on error. */
 int autotuneRenderBatch(const float* poses, size_t poseCount, int repeats, float* trials, size_t maxTrials);
 
 /* Bandwidth reference for profiling: copies `bytes` (rounded down to 16-byte
