@@ -34,6 +34,7 @@ struct EyeState {
   std::vector<uint32_t> counts;  // per-ommatidium sample counts, empty = uniform spp
   std::vector<uint32_t> offsets; // counts' exclusive prefix sum (size omms + 1)
   uint32_t frame = 0;
+  uint32_t sample_base = 0;      // random-stream index of each ommatidium's first sample
   uint32_t eye_index = 0;
   std::string data_path;
   std::string shader = "single_dimension_fast";
@@ -282,6 +283,7 @@ UnitLaunch unit_launch_of(EyeState& e, const Pose* d_poses, int pose_count) {
   U.seed = s.seed;
   U.eye = e.eye_index;
   U.frame = e.frame;
+  U.sample_base = e.sample_base;
   U.spp = e.spp;
   U.omm_counts = e.counts.empty() ? nullptr : e.d_counts;
   U.colours = e.d_colours;
@@ -885,6 +887,11 @@ void setCurrentEyeSampleCounts(const uint32_t* counts, size_t count) {
 uint64_t getCurrentEyeTotalSamples(void) {
   EyeState* e = current_eye();
   return e ? e->total_samples() : 0;
+}
+
+void setCurrentEyeSampleBase(uint32_t base) {
+  EyeState* e = current_eye();
+  if (e) e->sample_base = base;
 }
 
 void setRandomSeed(uint32_t seed) { st().seed = seed; }

@@ -137,6 +137,7 @@ def configure_extensions(lib):
     lib.setAdaptiveLabelError.argtypes = [c_float]
     lib.setAdaptiveLabelError.restype = c_int
     lib.setTraversalMode.argtypes = [c_int]
+    lib.setCurrentEyeSampleBase.argtypes = [c_uint32]
     lib.setSampleRecording.argtypes = [c_bool]
     lib.setRenderTuning.argtypes = [c_int, c_int]
     lib.getRenderTuning.argtypes = [i32p]

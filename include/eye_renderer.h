@@ -209,6 +209,10 @@ size_t copySceneSortOrder(uint32_t* ids, uint32_t* codes, size_t count);
 /* Rebuilds the BVH of the loaded scene with another leaf size (1..16). */
 int setSceneLeafSize(int leafSize);
 
+/* Random-stream index of the first sample of every ommatidium in renderFrame
+ * and renderPoseBatch (default 0): samples base .. base + n - 1 are drawn, so
+ * a later render can add samples independent of an earlier one. */
+void setCurrentEyeSampleBase(uint32_t base);
 /* Traversal strategy of eye rays: 0 = one ray per lane (own LDS stack),
  * 1 = wave packets (default), 2 = wave packets reading nodes and triangles
  * through the scalar unit.  Results are identical; only speed differs. */
