@@ -55,6 +55,11 @@ struct EyeState {
   size_t d_colours_cap = 0;
   bool has_render = false;
   uint64_t rendered_rays = 0;
+  // scratch of split renders (per-lane chunk values, per-unit counters)
+  float4* d_chunk_partial = nullptr;
+  size_t d_chunk_partial_cap = 0;
+  uint32_t* d_chunk_done = nullptr;
+  size_t d_chunk_done_cap = 0;
   // pixel -> ommatidium map of the spherical projections, valid for
   // (proj_w, proj_h, proj_variant) and the ommatidia of proj_version
   uint32_t* d_proj_ids = nullptr;
@@ -116,7 +121,7 @@ struct State {
   AdaptiveRenderer adaptive;
   bool record_samples = false;   // keep per-sample records of renderFrame
   uint32_t* d_queue = nullptr;   // work counter of the unit renderer
-  int units_per_grab = 4;
+  int units_per_grab = 0;        // 0: automatic
   int unit_blocks = 0;           // 0: automatic
   bool autotune = false;         // tune renderPoseBatch's launch shape per workload
   float label_error = 0.01f;     // error the sampler network's labels were made for
@@ -200,6 +205,8 @@ void free_scene() {
     if (e.d_samples) (void)hipFree(e.d_samples);
     if (e.d_colours) (void)hipFree(e.d_colours);
     if (e.d_proj_ids) (void)hipFree(e.d_proj_ids);
+    if (e.d_chunk_partial) (void)hipFree(e.d_chunk_partial);
+    if (e.d_chunk_done) (void)hipFree(e.d_chunk_done);
     if (e.d_counts) (void)hipFree(e.d_counts);
     if (e.d_sample_offsets) (void)hipFree(e.d_sample_offsets);
     if (e.d_order) (void)hipFree(e.d_order);
@@ -335,6 +342,16 @@ double render_compound(EyeState& e, const Cam& c) {
   if (record) {
     U.samples = e.d_samples;
     U.sample_offsets = e.d_sample_offsets;
+  }
+  if (e.counts.empty() && e.spp > 64) {
+    // one pose: let the chunks of each ommatidium's samples spread over waves
+    const size_t chunks = ((size_t)e.spp + 63) / 64;
+    ensure_device(&e.d_chunk_partial, &e.d_chunk_partial_cap, e.omms.size() * chunks * 64);
+    ensure_device(&e.d_chunk_done, &e.d_chunk_done_cap, e.omms.size());
+    U.chunk_partial = e.d_chunk_partial;
+    U.chunk_partial_cap = e.d_chunk_partial_cap;
+    U.chunk_done = e.d_chunk_done;
+    U.chunk_done_cap = e.d_chunk_done_cap;
   }
   if (!launch_render_units(s.scene, U, s.stream, &err)) throw std::runtime_error(err);
   hip_check(hipEventRecord(s.ev[1], s.stream), "event");
@@ -1116,7 +1133,9 @@ int setAutotuneCacheFile(const char* path) {
   State& s = st();
   s.tune_cache = path ? path : "";
   std::string err;
-  if (!s.tune_cache.empty() && !s.tuner.load(s.tune_cache, &err)) throw std::runtime_error(err);
+  // merge the file's entries with the tuner's, and keep the file up to date
+  if (!s.tune_cache.empty() && (!s.tuner.load(s.tune_cache, &err) || !s.tuner.save(s.tune_cache, &err)))
+    throw std::runtime_error(err);
   return (int)s.tuner.size();
   ABI_GUARD_END(return -1)
 }
@@ -1341,7 +1360,7 @@ void setSampleRecording(bool enabled) { st().record_samples = enabled; }
 
 void setRenderTuning(int unitsPerGrab, int blocks) {
   State& s = st();
-  s.units_per_grab = std::max(1, std::min(unitsPerGrab, 1024));
+  s.units_per_grab = std::max(0, std::min(unitsPerGrab, 1024));
   s.unit_blocks = std::max(0, blocks);
 }
 

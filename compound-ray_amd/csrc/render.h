@@ -49,8 +49,16 @@ struct UnitLaunch {
   float4* samples = nullptr;            // nullable: per-sample (r, g, b, t or -1)
   const uint64_t* sample_offsets = nullptr;  // first sample record of each unit
   uint32_t* queue = nullptr;            // device work counter
+  // Optional scratch for split launches (uniform sample counts above 64 and
+  // few units): each unit's 64-sample chunks go to different waves; per-lane
+  // chunk values (1 float4, 3 with stats) and a completion counter per unit.
+  float4* chunk_partial = nullptr;
+  uint64_t chunk_partial_cap = 0;       // float4 elements
+  uint32_t* chunk_done = nullptr;
+  uint64_t chunk_done_cap = 0;          // counters
   // launch shape (results do not depend on it)
-  int units_per_grab = 4;               // consecutive units a wave takes from the queue
+  int units_per_grab = 0;               // consecutive units a wave takes from the queue (0: chosen
+                                        //   from the launch size)
   int blocks = 0;                       // blocks of 256 threads; 0: every resident slot once
   int traversal = 0;                    // UnitTraversal
 };

@@ -485,6 +485,126 @@ __global__ __launch_bounds__(kTraceThreads) void k_render_units(SceneView S, Uni
   }
 }
 
+// Split launch of units with a uniform sample count n > 64: task t is chunk
+// t % C of unit t / C (C = ceil(n / 64)), so that a single pose's few units
+// still fill the GPU.  Lane l of a task keeps the terms of its one sample
+// (scaled colour, and the statistics terms) in `partial`; the wave that
+// completes a unit's last chunk adds the chunks lane by lane in chunk order and
+// then across the wave, which gives the same sums as k_render_units (there a
+// lane adds samples l, l + 64, ... in this order).
+template <bool kStats, bool kSamples, int kTrav>
+__global__ __launch_bounds__(kTraceThreads, 5) void k_render_chunks(SceneView S, UnitLaunch U, int wstride, int chunks) {
+  extern __shared__ int wave_stacks[];
+  const int lane = threadIdx.x & 63;
+  int* wst = wave_stacks + (threadIdx.x >> 6) * wstride;
+  uint32_t* lane_stack = reinterpret_cast<uint32_t*>(wave_stacks) + threadIdx.x;
+  constexpr int kPer = kStats ? 3 : 1;
+  const uint64_t units = (uint64_t)U.pose_count * (uint64_t)U.omm_count;
+  const uint64_t tasks = units * (uint64_t)chunks;
+  const uint32_t n = (uint32_t)U.spp;
+  const float inv = 1.0f / (float)n;
+  for (;;) {
+    uint32_t t = 0;
+    if (lane == 0) t = atomicAdd(U.queue, 1u);
+    t = __builtin_amdgcn_readfirstlane(__shfl(t, 0));
+    if ((uint64_t)t >= tasks) break;
+    const uint64_t uq = (uint64_t)t / (uint64_t)chunks;
+    const uint32_t j = t - (uint32_t)(uq * (uint64_t)chunks);
+    int p, i;
+    uint64_t out;
+    locate_unit(U, uq, &p, &i, &out);
+    const uint32_t s = j * 64u + (uint32_t)lane;
+    const bool active = s < n;
+    Ray ray;
+    if (active) {
+      const U4 rnd = sample_randoms(U.seed, U.eye, (uint32_t)i, U.sample_base + s, U.frame,
+                                    U.pose_base + (uint32_t)p);
+      ray = generate_ray(U.omms[i], U.poses[p], rnd);
+    } else {
+      ray.ox = ray.oy = ray.oz = 0.0f;
+      ray.dx = ray.dy = 0.0f;
+      ray.dz = 1.0f;
+      ray.tmin = 0.0f;
+      ray.tmax = 1.0f;
+    }
+    const RayPre rp = precompute_ray(ray);
+    Hit h;
+    if (kTrav == kUnitPerRay) {
+      h.t = rp.tmax;
+      h.k = h.id = 0xffffffffu;
+      h.b1 = h.b2 = 0.0f;
+      if (active) closest_hit(S, rp, lane_stack, (int)blockDim.x, h);
+    } else {
+      closest_hit_packet<kTrav == kUnitPacketScalar>(S, rp, active, wst, h);
+    }
+    float4 a = make_float4(0.0f, 0.0f, 0.0f, 0.0f), b = a, c2 = a;
+    if (active) {
+      const V3 c = shade(S, h, ray.dx, ray.dy, ray.dz);
+      const bool hit = h.id != 0xffffffffu;
+      a = make_float4(c.x * inv, c.y * inv, c.z * inv, 0.0f);
+      if (kSamples) U.samples[U.sample_offsets[out] + s] = make_float4(c.x, c.y, c.z, hit ? h.t : -1.0f);
+      if (kStats) {
+        const float idp = hit ? 1.0f / (1.0f + h.t) : 0.0f;
+        b = make_float4(c.x * c.x, c.y * c.y, c.z * c.z, idp);
+        c2 = make_float4(idp * idp, hit ? 1.0f : 0.0f, 0.0f, 0.0f);
+      }
+    }
+    float4* slot = U.chunk_partial + ((uq * (uint64_t)chunks + j) * 64u + (uint64_t)lane) * kPer;
+    slot[0] = a;
+    if (kStats) {
+      slot[1] = b;
+      slot[2] = c2;
+    }
+    __threadfence();
+    uint32_t prev = 0;
+    if (lane == 0) prev = atomicAdd(U.chunk_done + uq, 1u);
+    prev = __builtin_amdgcn_readfirstlane(__shfl(prev, 0));
+    if (prev != (uint32_t)chunks - 1u) continue;
+    __threadfence();
+    float r = 0.0f, gg = 0.0f, bb = 0.0f;
+    float sr = 0.0f, sg = 0.0f, sb = 0.0f, sd = 0.0f, sd2 = 0.0f, sh = 0.0f;
+    for (int q = 0; q < chunks; ++q) {
+      const float4* src = U.chunk_partial + ((uq * (uint64_t)chunks + (uint64_t)q) * 64u + (uint64_t)lane) * kPer;
+      if (q * 64 + lane >= (int)n) break;
+      const float4 v = src[0];
+      r = r + v.x;
+      gg = gg + v.y;
+      bb = bb + v.z;
+      if (kStats) {
+        const float4 w = src[1], x = src[2];
+        sr += w.x;
+        sg += w.y;
+        sb += w.z;
+        sd += w.w;
+        sd2 += x.x;
+        sh += x.y;
+      }
+    }
+    r = segment_sum(r, 64);
+    gg = segment_sum(gg, 64);
+    bb = segment_sum(bb, 64);
+    if (kStats) {
+      sr = segment_sum(sr, 64);
+      sg = segment_sum(sg, 64);
+      sb = segment_sum(sb, 64);
+      sd = segment_sum(sd, 64);
+      sd2 = segment_sum(sd2, 64);
+      sh = segment_sum(sh, 64);
+    }
+    if (lane == 0) {
+      U.colours[out] = make_float4(r, gg, bb, (float)n);
+      if (kStats) {
+        const float vr = fmaxf(sr * inv - r * r, 0.0f);
+        const float vg = fmaxf(sg * inv - gg * gg, 0.0f);
+        const float vb = fmaxf(sb * inv - bb * bb, 0.0f);
+        const float md = sd * inv;
+        U.stats[2 * out] = make_float4(vr, vg, vb, sh * inv);
+        U.stats[2 * out + 1] = make_float4(md, fmaxf(sd2 * inv - md * md, 0.0f), 0.0f, 0.0f);
+      }
+    }
+  }
+}
+
 __global__ void k_fill_u32(uint32_t* __restrict__ p, uint32_t n, uint32_t v) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
@@ -628,6 +748,20 @@ static void launch_units_variant(const UnitLaunch& U, const SceneView& v, int bl
 }
 
 template <int kTrav>
+static void launch_chunks_variant(const UnitLaunch& U, const SceneView& v, int blocks, size_t lds, int wstride,
+                                  int chunks, hipStream_t stream) {
+  const dim3 grid(blocks), block(kTraceThreads);
+  if (U.stats && U.samples)
+    hipLaunchKernelGGL((k_render_chunks<true, true, kTrav>), grid, block, lds, stream, v, U, wstride, chunks);
+  else if (U.stats)
+    hipLaunchKernelGGL((k_render_chunks<true, false, kTrav>), grid, block, lds, stream, v, U, wstride, chunks);
+  else if (U.samples)
+    hipLaunchKernelGGL((k_render_chunks<false, true, kTrav>), grid, block, lds, stream, v, U, wstride, chunks);
+  else
+    hipLaunchKernelGGL((k_render_chunks<false, false, kTrav>), grid, block, lds, stream, v, U, wstride, chunks);
+}
+
+template <int kTrav>
 static const void* units_kernel_variant(const UnitLaunch& U) {
   if (U.stats && U.samples) return reinterpret_cast<const void*>(&k_render_units<true, true, kTrav>);
   if (U.stats) return reinterpret_cast<const void*>(&k_render_units<true, false, kTrav>);
@@ -675,7 +809,7 @@ bool launch_render_units(const DeviceScene& scene, const UnitLaunch& U, hipStrea
     if (err) *err = "render_units: too many units for one launch";
     return false;
   }
-  if (!U.queue || !U.colours || U.units_per_grab < 1 || (U.samples && !U.sample_offsets)) {
+  if (!U.queue || !U.colours || (U.samples && !U.sample_offsets)) {
     if (err) *err = "render_units: incomplete launch description";
     return false;
   }
@@ -689,18 +823,46 @@ bool launch_render_units(const DeviceScene& scene, const UnitLaunch& U, hipStrea
     if (err) *err = "render_units: BVH too deep for the LDS traversal stack";
     return false;
   }
-  int blocks = U.blocks > 0 ? U.blocks : render_units_capacity(scene, U);
-  const uint64_t need = (units + (uint64_t)U.units_per_grab - 1) / (uint64_t)U.units_per_grab;
+  const int capacity = render_units_capacity(scene, U);
+  int blocks = U.blocks > 0 ? U.blocks : capacity;
+  const SceneView v = view_of(scene);
+  // Split launch: uniform counts above 64 samples and fewer units than two per
+  // resident wave (a single pose, typically), when the caller gave scratch.
+  const uint64_t chunks = U.spp > 64 ? ((uint64_t)U.spp + 63u) / 64u : 1u;
+  const uint64_t resident_waves = (uint64_t)capacity * (kTraceThreads / 64);
+  const uint64_t per_chunk = U.stats ? 3u : 1u;
+  if (!U.omm_counts && !U.unit_counts && chunks > 1 && units < 2 * resident_waves && U.chunk_partial &&
+      U.chunk_done && units * chunks <= 0xffffffffull - 65536ull &&
+      units * chunks * 64u * per_chunk <= U.chunk_partial_cap && units <= U.chunk_done_cap) {
+    const uint64_t tasks = units * chunks;
+    const uint64_t waves = (uint64_t)blocks * (kTraceThreads / 64);
+    if (waves > tasks) blocks = (int)((tasks + (kTraceThreads / 64) - 1) / (kTraceThreads / 64));
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, stream, U.queue, 1u, 0u);
+    hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((units + 255) / 256)), dim3(256), 0, stream, U.chunk_done,
+                       (uint32_t)units, 0u);
+    if (U.traversal == kUnitPerRay)
+      launch_chunks_variant<kUnitPerRay>(U, v, blocks, lds, wstride, (int)chunks, stream);
+    else if (U.traversal == kUnitPacketScalar)
+      launch_chunks_variant<kUnitPacketScalar>(U, v, blocks, lds, wstride, (int)chunks, stream);
+    else
+      launch_chunks_variant<kUnitPacket>(U, v, blocks, lds, wstride, (int)chunks, stream);
+    return check_launch("render_chunks", err);
+  }
+  // units per queue grab: as given, or (0) as many as keep two grabs per
+  // resident wave, at most 16
+  UnitLaunch G = U;
+  if (G.units_per_grab <= 0)
+    G.units_per_grab = (int)std::min<uint64_t>(16, std::max<uint64_t>(1, units / (2 * resident_waves)));
+  const uint64_t need = (units + (uint64_t)G.units_per_grab - 1) / (uint64_t)G.units_per_grab;
   const uint64_t waves = (uint64_t)blocks * (kTraceThreads / 64);
   if (waves > need) blocks = (int)((need + (kTraceThreads / 64) - 1) / (kTraceThreads / 64));
-  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, stream, U.queue, 1u, 0u);
-  const SceneView v = view_of(scene);
-  if (U.traversal == kUnitPerRay)
-    launch_units_variant<kUnitPerRay>(U, v, blocks, lds, wstride, stream);
-  else if (U.traversal == kUnitPacketScalar)
-    launch_units_variant<kUnitPacketScalar>(U, v, blocks, lds, wstride, stream);
+  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, stream, G.queue, 1u, 0u);
+  if (G.traversal == kUnitPerRay)
+    launch_units_variant<kUnitPerRay>(G, v, blocks, lds, wstride, stream);
+  else if (G.traversal == kUnitPacketScalar)
+    launch_units_variant<kUnitPacketScalar>(G, v, blocks, lds, wstride, stream);
   else
-    launch_units_variant<kUnitPacket>(U, v, blocks, lds, wstride, stream);
+    launch_units_variant<kUnitPacket>(G, v, blocks, lds, wstride, stream);
   return check_launch("render_units", err);
 }
 

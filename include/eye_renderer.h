@@ -230,8 +230,9 @@ int getRenderTuning(int* shape3);
  * workload (scene, eye, sample counts) times the candidate shapes on a sample
  * of its poses and later batches reuse the winner. */
 void setBatchAutotune(int enabled);
-/* Loads tuned shapes from a text file and saves later tunes there; returns
- * the number of cached entries, -1 on a malformed file. */
+/* Merges the tuned shapes of a text file with those already known, writes
+ * them all back and saves later tunes there (NULL: stop saving); returns the
+ * number of cached entries, -1 on a malformed or unwritable file. */
 int setAutotuneCacheFile(const char* path);
 /* Times every candidate shape on (a sample of) these poses and installs the
  * fastest as the shape of all later unit launches.  trials receives rows of

@@ -48,7 +48,10 @@ def test_camera_rays_and_frame(lib, camera_scene, oracle_c1, index, kind):
     w, h = 96, 64
     er.setRenderSize(lib, w, h)
     try:
-        lib.setCameraPose(0.1 * index, float(cam.translation[1]), -0.2, -0.35, 0.5 + index, 0.0)
+        # pitched so that every camera's view holds terrain and sky (the
+        # orthographic camera looks along its +z axis, the others around it)
+        pitch = 0.6 if kind == 1 else -0.35
+        lib.setCameraPose(0.1 * index, float(cam.translation[1]), -0.2, pitch, 0.5 + index, 0.0)
         pose = er.pose_np(lib)
         scale = pinhole_scale(cam.yfov) if kind == 0 else np.array([cam.xmag, cam.ymag, 0.0], np.float32)
         want = oracle.camera_rays(kind, pose, scale, w, h)
