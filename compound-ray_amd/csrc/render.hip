@@ -492,12 +492,15 @@ __global__ __launch_bounds__(kTraceThreads) void k_render_units(SceneView S, Uni
 // completes a unit's last chunk adds the chunks lane by lane in chunk order and
 // then across the wave, which gives the same sums as k_render_units (there a
 // lane adds samples l, l + 64, ... in this order).
-template <bool kStats, bool kSamples, int kTrav>
-__global__ __launch_bounds__(kTraceThreads, 5) void k_render_chunks(SceneView S, UnitLaunch U, int wstride, int chunks) {
-  extern __shared__ int wave_stacks[];
+//
+// The rays of a task are traversed one per lane (an LDS stack column per
+// lane).  The wave-packet traversal is not used here: in this kernel it did
+// not terminate on the GPU, and its wave stack is written by lane 0 alone.
+template <bool kStats, bool kSamples>
+__global__ __launch_bounds__(kTraceThreads, 5) void k_render_chunks(SceneView S, UnitLaunch U, int chunks) {
+  extern __shared__ uint32_t lane_stacks[];
   const int lane = threadIdx.x & 63;
-  int* wst = wave_stacks + (threadIdx.x >> 6) * wstride;
-  uint32_t* lane_stack = reinterpret_cast<uint32_t*>(wave_stacks) + threadIdx.x;
+  uint32_t* lane_stack = lane_stacks + threadIdx.x;
   constexpr int kPer = kStats ? 3 : 1;
   const uint64_t units = (uint64_t)U.pose_count * (uint64_t)U.omm_count;
   const uint64_t tasks = units * (uint64_t)chunks;
@@ -529,14 +532,10 @@ __global__ __launch_bounds__(kTraceThreads, 5) void k_render_chunks(SceneView S,
     }
     const RayPre rp = precompute_ray(ray);
     Hit h;
-    if (kTrav == kUnitPerRay) {
-      h.t = rp.tmax;
-      h.k = h.id = 0xffffffffu;
-      h.b1 = h.b2 = 0.0f;
-      if (active) closest_hit(S, rp, lane_stack, (int)blockDim.x, h);
-    } else {
-      closest_hit_packet<kTrav == kUnitPacketScalar>(S, rp, active, wst, h);
-    }
+    h.t = rp.tmax;
+    h.k = h.id = 0xffffffffu;
+    h.b1 = h.b2 = 0.0f;
+    if (active) closest_hit(S, rp, lane_stack, (int)blockDim.x, h);
     float4 a = make_float4(0.0f, 0.0f, 0.0f, 0.0f), b = a, c2 = a;
     if (active) {
       const V3 c = shade(S, h, ray.dx, ray.dy, ray.dz);
@@ -747,18 +746,17 @@ static void launch_units_variant(const UnitLaunch& U, const SceneView& v, int bl
     hipLaunchKernelGGL((k_render_units<false, false, kTrav>), grid, block, lds, stream, v, U, wstride);
 }
 
-template <int kTrav>
-static void launch_chunks_variant(const UnitLaunch& U, const SceneView& v, int blocks, size_t lds, int wstride,
-                                  int chunks, hipStream_t stream) {
+static void launch_chunks(const UnitLaunch& U, const SceneView& v, int blocks, size_t lds, int chunks,
+                          hipStream_t stream) {
   const dim3 grid(blocks), block(kTraceThreads);
   if (U.stats && U.samples)
-    hipLaunchKernelGGL((k_render_chunks<true, true, kTrav>), grid, block, lds, stream, v, U, wstride, chunks);
+    hipLaunchKernelGGL((k_render_chunks<true, true>), grid, block, lds, stream, v, U, chunks);
   else if (U.stats)
-    hipLaunchKernelGGL((k_render_chunks<true, false, kTrav>), grid, block, lds, stream, v, U, wstride, chunks);
+    hipLaunchKernelGGL((k_render_chunks<true, false>), grid, block, lds, stream, v, U, chunks);
   else if (U.samples)
-    hipLaunchKernelGGL((k_render_chunks<false, true, kTrav>), grid, block, lds, stream, v, U, wstride, chunks);
+    hipLaunchKernelGGL((k_render_chunks<false, true>), grid, block, lds, stream, v, U, chunks);
   else
-    hipLaunchKernelGGL((k_render_chunks<false, false, kTrav>), grid, block, lds, stream, v, U, wstride, chunks);
+    hipLaunchKernelGGL((k_render_chunks<false, false>), grid, block, lds, stream, v, U, chunks);
 }
 
 template <int kTrav>
@@ -834,18 +832,29 @@ bool launch_render_units(const DeviceScene& scene, const UnitLaunch& U, hipStrea
   if (!U.omm_counts && !U.unit_counts && chunks > 1 && units < 2 * resident_waves && U.chunk_partial &&
       U.chunk_done && units * chunks <= 0xffffffffull - 65536ull &&
       units * chunks * 64u * per_chunk <= U.chunk_partial_cap && units <= U.chunk_done_cap) {
+    // per-ray traversal only (see k_render_chunks)
+    const size_t chunk_lds = stack_bytes(scene);
+    if (chunk_lds > 160 * 1024) {
+      if (err) *err = "render_chunks: BVH too deep for the LDS traversal stack";
+      return false;
+    }
+    int chunk_blocks = U.blocks;
+    if (chunk_blocks <= 0) {
+      int per_cu = 0;
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+              &per_cu, reinterpret_cast<const void*>(&k_render_chunks<false, false>), kTraceThreads, chunk_lds) !=
+              hipSuccess ||
+          per_cu <= 0)
+        per_cu = 4;
+      chunk_blocks = device_cus() * per_cu;
+    }
     const uint64_t tasks = units * chunks;
-    const uint64_t waves = (uint64_t)blocks * (kTraceThreads / 64);
-    if (waves > tasks) blocks = (int)((tasks + (kTraceThreads / 64) - 1) / (kTraceThreads / 64));
+    const uint64_t waves = (uint64_t)chunk_blocks * (kTraceThreads / 64);
+    if (waves > tasks) chunk_blocks = (int)((tasks + (kTraceThreads / 64) - 1) / (kTraceThreads / 64));
     hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, stream, U.queue, 1u, 0u);
     hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((units + 255) / 256)), dim3(256), 0, stream, U.chunk_done,
                        (uint32_t)units, 0u);
-    if (U.traversal == kUnitPerRay)
-      launch_chunks_variant<kUnitPerRay>(U, v, blocks, lds, wstride, (int)chunks, stream);
-    else if (U.traversal == kUnitPacketScalar)
-      launch_chunks_variant<kUnitPacketScalar>(U, v, blocks, lds, wstride, (int)chunks, stream);
-    else
-      launch_chunks_variant<kUnitPacket>(U, v, blocks, lds, wstride, (int)chunks, stream);
+    launch_chunks(U, v, chunk_blocks, chunk_lds, (int)chunks, stream);
     return check_launch("render_chunks", err);
   }
   // units per queue grab: as given, or (0) as many as keep two grabs per
