@@ -34,11 +34,11 @@ METRIC = "eye-frames/s at ≤1% RMS colour err; Grays/s; learned-adaptive vs uni
 HBM_PEAK_GBS = 8000.0
 
 
-TRAFFIC_FILE = ROOT / "profiles" / "k_render_units_c2_traffic.txt"
+TRAFFIC_FILE = ROOT / "profiles" / "k_render_chunks_c2_traffic.txt"
 
 
 def measured_traffic():
-    """HBM bytes per k_render_units launch on this workload, from the committed
+    """HBM bytes per render launch on this workload, from the committed
     rocprofv3 FETCH_SIZE / WRITE_SIZE summary (already scaled by the copy-kernel
     factors recorded there); None when no summary is present."""
     try:
@@ -424,6 +424,7 @@ def main() -> int:
     ap.add_argument("--skip-adaptive", action="store_true", help="skip the learned adaptive sampling comparison")
     ap.add_argument("--large", action="store_true",
                     help="also run the 256-eye, 13.4M-triangle heterogeneous batch (several minutes)")
+    ap.add_argument("--no-split", action="store_true", help="disable split launches (chunks of a unit on several waves)")
     ap.add_argument("--trace-mode", type=int, default=1, choices=(0, 1, 2),
                     help="traversal: 0 per-ray, 1 wave packets (default), 2 wave packets with scalar node loads")
     ap.add_argument("--copy-probe", type=float, default=0.0,
@@ -440,6 +441,7 @@ def main() -> int:
     lib = cra.load_library()
     lib.setVerbosity(False)
     lib.setTraversalMode(args.trace_mode)
+    lib.setSplitLaunches(not args.no_split)
     tmp = tempfile.mkdtemp(prefix="eyebench-")
     t0 = time.time()
     cfg = scenes.build_config(2, tmp)
@@ -480,6 +482,7 @@ def main() -> int:
     for _ in range(args.warmup):
         step()
     trace_ms = []
+    kernel_name = ""
     t0 = time.perf_counter()
     for _ in range(args.steps):
         for i in eyes:
@@ -487,6 +490,7 @@ def main() -> int:
             if lib.renderFrame() < 0:
                 raise RuntimeError(er.last_error(lib))
             trace_ms.append(er.timings_np(lib)[0])
+            kernel_name = lib.getLastRenderKernel().decode()
     elapsed = time.perf_counter() - t0
     eye_frames = args.steps * len(eyes)
     value = eye_frames / elapsed
@@ -543,7 +547,7 @@ def main() -> int:
                               "hit_fraction": stats["hits"] / rays_per_step},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(),
-                     "kernel": "k_render_units", "bytes_per_launch": bytes_launch,
+                     "kernel": kernel_name, "bytes_per_launch": bytes_launch,
                      "trace_mode": args.trace_mode},
     }
     if not (args.no_cpu_baseline or args.profile):

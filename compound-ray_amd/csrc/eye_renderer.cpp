@@ -1358,6 +1358,10 @@ void setTraversalMode(int mode) {
 
 void setSampleRecording(bool enabled) { st().record_samples = enabled; }
 
+const char* getLastRenderKernel(void) { return last_render_kernel(); }
+
+void setSplitLaunches(bool enabled) { set_split_launches(enabled); }
+
 void setRenderTuning(int unitsPerGrab, int blocks) {
   State& s = st();
   s.units_per_grab = std::max(0, std::min(unitsPerGrab, 1024));

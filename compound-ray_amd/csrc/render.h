@@ -72,6 +72,11 @@ enum UnitTraversal : int { kUnitPacket = 0, kUnitPacketScalar = 1, kUnitPerRay =
 // cleared by a kernel first.
 bool launch_render_units(const DeviceScene& scene, const UnitLaunch& U, hipStream_t stream, std::string* err);
 
+// Name of the kernel the last launch_render_units used (k_render_units or the
+// split-launch k_render_chunks), and the switch for split launches (on by default).
+const char* last_render_kernel();
+void set_split_launches(bool on);
+
 // Blocks of the unit renderer that fit on the device at once (CUs x resident
 // blocks per CU for this scene's stack size and U's kernel variant).
 int render_units_capacity(const DeviceScene& scene, const UnitLaunch& U);

@@ -731,6 +731,8 @@ size_t stack_bytes(const DeviceScene& s) {
 }  // namespace
 
 static int g_trace_mode = kTracePacket;
+static const char* g_last_kernel = "";
+static bool g_split = true;
 
 template <int kTrav>
 static void launch_units_variant(const UnitLaunch& U, const SceneView& v, int blocks, size_t lds, int wstride,
@@ -829,7 +831,7 @@ bool launch_render_units(const DeviceScene& scene, const UnitLaunch& U, hipStrea
   const uint64_t chunks = U.spp > 64 ? ((uint64_t)U.spp + 63u) / 64u : 1u;
   const uint64_t resident_waves = (uint64_t)capacity * (kTraceThreads / 64);
   const uint64_t per_chunk = U.stats ? 3u : 1u;
-  if (!U.omm_counts && !U.unit_counts && chunks > 1 && units < 2 * resident_waves && U.chunk_partial &&
+  if (g_split && !U.omm_counts && !U.unit_counts && chunks > 1 && units < 2 * resident_waves && U.chunk_partial &&
       U.chunk_done && units * chunks <= 0xffffffffull - 65536ull &&
       units * chunks * 64u * per_chunk <= U.chunk_partial_cap && units <= U.chunk_done_cap) {
     // per-ray traversal only (see k_render_chunks)
@@ -855,6 +857,7 @@ bool launch_render_units(const DeviceScene& scene, const UnitLaunch& U, hipStrea
     hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((units + 255) / 256)), dim3(256), 0, stream, U.chunk_done,
                        (uint32_t)units, 0u);
     launch_chunks(U, v, chunk_blocks, chunk_lds, (int)chunks, stream);
+    g_last_kernel = "k_render_chunks";
     return check_launch("render_chunks", err);
   }
   // units per queue grab: as given, or (0) as many as keep two grabs per
@@ -872,8 +875,12 @@ bool launch_render_units(const DeviceScene& scene, const UnitLaunch& U, hipStrea
     launch_units_variant<kUnitPacketScalar>(G, v, blocks, lds, wstride, stream);
   else
     launch_units_variant<kUnitPacket>(G, v, blocks, lds, wstride, stream);
+  g_last_kernel = "k_render_units";
   return check_launch("render_units", err);
 }
+
+const char* last_render_kernel() { return g_last_kernel; }
+void set_split_launches(bool on) { g_split = on; }
 
 void set_trace_mode(int mode) { g_trace_mode = mode; }
 int trace_mode() { return g_trace_mode; }

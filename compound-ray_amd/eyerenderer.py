@@ -140,6 +140,8 @@ def configure_extensions(lib):
     lib.setCurrentEyeSampleBase.argtypes = [c_uint32]
     lib.setSampleRecording.argtypes = [c_bool]
     lib.setRenderTuning.argtypes = [c_int, c_int]
+    lib.getLastRenderKernel.restype = c_char_p
+    lib.setSplitLaunches.argtypes = [c_bool]
     lib.getRenderTuning.argtypes = [i32p]
     lib.getRenderTuning.restype = c_int
     lib.setBatchAutotune.argtypes = [c_int]

@@ -220,6 +220,12 @@ void setTraversalMode(int mode);
 /* Keep the per-sample records of renderFrame for getCurrentEyeSamples (off by
  * default: the packet renderer reduces samples in registers). */
 void setSampleRecording(bool enabled);
+/* Kernel of the last compound-eye render: "k_render_units", or
+ * "k_render_chunks" for a split launch (uniform counts above 64 samples and
+ * few units, where each unit's 64-sample chunks run on different waves). */
+const char* getLastRenderKernel(void);
+/* Allows split launches (default true); results are the same either way. */
+void setSplitLaunches(bool enabled);
 /* Launch shape of the persistent unit renderer: ommatidia taken from the work
  * queue per grab, and blocks of 256 threads (0 = every resident slot once). */
 void setRenderTuning(int unitsPerGrab, int blocks);
