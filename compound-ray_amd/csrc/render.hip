@@ -604,6 +604,74 @@ __global__ __launch_bounds__(kTraceThreads, 5) void k_render_chunks(SceneView S,
   }
 }
 
+// Non-persistent form of the split launch for wave-packet traversal: wave w
+// of the grid is task w (chunk w % C of queue position w / C), the whole wave
+// returns together when w is past the last task, and nothing but the
+// traversal sits between the ray generation and the chunk store, which keeps
+// register use low (more resident waves).  Same per-lane chunk values,
+// completion counters and last-wave reduction as k_render_chunks (no
+// statistics).
+template <bool kSamples, bool kScalar>
+__global__ __launch_bounds__(kTraceThreads, 6) void k_render_tasks(SceneView S, UnitLaunch U, int wstride, int chunks) {
+  extern __shared__ int wave_stacks[];
+  const int lane = threadIdx.x & 63;
+  int* wst = wave_stacks + (threadIdx.x >> 6) * wstride;
+  const uint64_t units = (uint64_t)U.pose_count * (uint64_t)U.omm_count;
+  const uint64_t tasks = units * (uint64_t)chunks;
+  const uint64_t t = (uint64_t)blockIdx.x * (kTraceThreads / 64) + (uint64_t)(threadIdx.x >> 6);
+  if (t >= tasks) return;  // whole wave
+  const uint32_t n = (uint32_t)U.spp;
+  const float inv = 1.0f / (float)n;
+  const uint64_t uq = t / (uint64_t)chunks;
+  const uint32_t j = (uint32_t)(t - uq * (uint64_t)chunks);
+  int p, i;
+  uint64_t out;
+  locate_unit(U, uq, &p, &i, &out);
+  const uint32_t s = j * 64u + (uint32_t)lane;
+  const bool active = s < n;
+  Ray ray;
+  if (active) {
+    const U4 rnd = sample_randoms(U.seed, U.eye, (uint32_t)i, U.sample_base + s, U.frame,
+                                  U.pose_base + (uint32_t)p);
+    ray = generate_ray(U.omms[i], U.poses[p], rnd);
+  } else {
+    ray.ox = ray.oy = ray.oz = 0.0f;
+    ray.dx = ray.dy = 0.0f;
+    ray.dz = 1.0f;
+    ray.tmin = 0.0f;
+    ray.tmax = 1.0f;
+  }
+  const RayPre rp = precompute_ray(ray);
+  Hit h;
+  closest_hit_packet<kScalar>(S, rp, active, wst, h);
+  float4 a = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  if (active) {
+    const V3 c = shade(S, h, ray.dx, ray.dy, ray.dz);
+    a = make_float4(c.x * inv, c.y * inv, c.z * inv, 0.0f);
+    if (kSamples)
+      U.samples[U.sample_offsets[out] + s] = make_float4(c.x, c.y, c.z, h.id != 0xffffffffu ? h.t : -1.0f);
+  }
+  U.chunk_partial[(uq * (uint64_t)chunks + j) * 64u + (uint64_t)lane] = a;
+  __threadfence();
+  uint32_t prev = 0;
+  if (lane == 0) prev = atomicAdd(U.chunk_done + uq, 1u);
+  prev = __builtin_amdgcn_readfirstlane(__shfl(prev, 0));
+  if (prev != (uint32_t)chunks - 1u) return;  // whole wave
+  __threadfence();
+  float r = 0.0f, gg = 0.0f, bb = 0.0f;
+  for (int q = 0; q < chunks; ++q) {
+    if (q * 64 + lane >= (int)n) break;
+    const float4 v = U.chunk_partial[(uq * (uint64_t)chunks + (uint64_t)q) * 64u + (uint64_t)lane];
+    r = r + v.x;
+    gg = gg + v.y;
+    bb = bb + v.z;
+  }
+  r = segment_sum(r, 64);
+  gg = segment_sum(gg, 64);
+  bb = segment_sum(bb, 64);
+  if (lane == 0) U.colours[out] = make_float4(r, gg, bb, (float)n);
+}
+
 __global__ void k_fill_u32(uint32_t* __restrict__ p, uint32_t n, uint32_t v) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
@@ -834,7 +902,25 @@ bool launch_render_units(const DeviceScene& scene, const UnitLaunch& U, hipStrea
   if (g_split && !U.omm_counts && !U.unit_counts && chunks > 1 && units < 2 * resident_waves && U.chunk_partial &&
       U.chunk_done && units * chunks <= 0xffffffffull - 65536ull &&
       units * chunks * 64u * per_chunk <= U.chunk_partial_cap && units <= U.chunk_done_cap) {
-    // per-ray traversal only (see k_render_chunks)
+    const uint64_t tasks = units * chunks;
+    if (U.traversal != kUnitPerRay && !U.stats && tasks / (kTraceThreads / 64) < 0x7fffffffull) {
+      // wave packets: the non-persistent task kernel, one wave per chunk
+      hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((units + 255) / 256)), dim3(256), 0, stream, U.chunk_done,
+                         (uint32_t)units, 0u);
+      const dim3 grid((unsigned)((tasks + (kTraceThreads / 64) - 1) / (kTraceThreads / 64))), block(kTraceThreads);
+      const bool scalar = U.traversal == kUnitPacketScalar;
+      if (U.samples && scalar)
+        hipLaunchKernelGGL((k_render_tasks<true, true>), grid, block, lds, stream, v, U, wstride, (int)chunks);
+      else if (U.samples)
+        hipLaunchKernelGGL((k_render_tasks<true, false>), grid, block, lds, stream, v, U, wstride, (int)chunks);
+      else if (scalar)
+        hipLaunchKernelGGL((k_render_tasks<false, true>), grid, block, lds, stream, v, U, wstride, (int)chunks);
+      else
+        hipLaunchKernelGGL((k_render_tasks<false, false>), grid, block, lds, stream, v, U, wstride, (int)chunks);
+      g_last_kernel = "k_render_tasks";
+      return check_launch("render_tasks", err);
+    }
+    // per-ray traversal: the persistent chunk kernel
     const size_t chunk_lds = stack_bytes(scene);
     if (chunk_lds > 160 * 1024) {
       if (err) *err = "render_chunks: BVH too deep for the LDS traversal stack";
@@ -850,7 +936,6 @@ bool launch_render_units(const DeviceScene& scene, const UnitLaunch& U, hipStrea
         per_cu = 4;
       chunk_blocks = device_cus() * per_cu;
     }
-    const uint64_t tasks = units * chunks;
     const uint64_t waves = (uint64_t)chunk_blocks * (kTraceThreads / 64);
     if (waves > tasks) chunk_blocks = (int)((tasks + (kTraceThreads / 64) - 1) / (kTraceThreads / 64));
     hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, stream, U.queue, 1u, 0u);
