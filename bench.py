@@ -34,15 +34,17 @@ METRIC = "eye-frames/s at ≤1% RMS colour err; Grays/s; learned-adaptive vs uni
 HBM_PEAK_GBS = 8000.0
 
 
-TRAFFIC_FILE = ROOT / "profiles" / "k_render_chunks_c2_traffic.txt"
+PROFILES = ROOT / "profiles"
 
 
-def measured_traffic():
-    """HBM bytes per render launch on this workload, from the committed
-    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (already scaled by the copy-kernel
-    factors recorded there); None when no summary is present."""
+def measured_traffic(kernel: str):
+    """HBM bytes per launch of `kernel` on the bee-scale workload, from the
+    committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary named for that kernel
+    (already scaled by the copy-kernel factors recorded there); None when no
+    summary of this kernel is present."""
+    base = kernel.split("<")[0]
     try:
-        for line in TRAFFIC_FILE.read_text().splitlines():
+        for line in (PROFILES / f"{base}_c2_traffic.txt").read_text().splitlines():
             if line.startswith("traffic_bytes_per_launch:"):
                 return float(line.split(":", 1)[1])
     except (OSError, ValueError):
@@ -546,7 +548,7 @@ def main() -> int:
                               "tris": stats["tris_tested"] / rays_per_step,
                               "hit_fraction": stats["hits"] / rays_per_step},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(),
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(kernel_name),
                      "kernel": kernel_name, "bytes_per_launch": bytes_launch,
                      "trace_mode": args.trace_mode},
     }

@@ -46,8 +46,13 @@ __host__ __device__ inline float fkey_inv(uint32_t k) {
   return f;
 }
 
-__global__ void k_tri_bounds(const float* __restrict__ pos, int n, float* __restrict__ tb,
-                             float* __restrict__ cen, uint32_t* __restrict__ cbounds) {
+// Triangle boxes and centroids; each block also writes the order-preserving
+// keys of its centroid box to part[6 * blockIdx.x ...] (lo x,y,z, hi x,y,z),
+// reduced by k_centroid_box.  No atomics: the six global addresses would
+// otherwise serialise one atomic per wave.
+__global__ __launch_bounds__(256) void k_tri_bounds(const float* __restrict__ pos, int n, float* __restrict__ tb,
+                                                    float* __restrict__ cen, uint32_t* __restrict__ part) {
+  __shared__ uint32_t wave_keys[4][6];
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   uint32_t klo[3] = {0xffffffffu, 0xffffffffu, 0xffffffffu};
   uint32_t khi[3] = {0u, 0u, 0u};
@@ -64,7 +69,7 @@ __global__ void k_tri_bounds(const float* __restrict__ pos, int n, float* __rest
       khi[a] = fkey(c);
     }
   }
-  // wave reduction, then one atomic per wave and axis
+  const int wave = threadIdx.x >> 6;
   for (int a = 0; a < 3; ++a) {
     uint32_t lo = klo[a], hi = khi[a];
     for (int m = 32; m >= 1; m >>= 1) {
@@ -74,9 +79,49 @@ __global__ void k_tri_bounds(const float* __restrict__ pos, int n, float* __rest
       hi = ohi > hi ? ohi : hi;
     }
     if ((threadIdx.x & 63) == 0) {
-      atomicMin(&cbounds[a], lo);
-      atomicMax(&cbounds[3 + a], hi);
+      wave_keys[wave][a] = lo;
+      wave_keys[wave][3 + a] = hi;
     }
+  }
+  __syncthreads();
+  if (threadIdx.x < 6) {
+    const int a = threadIdx.x;
+    uint32_t k = wave_keys[0][a];
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) {
+      const uint32_t o = wave_keys[w][a];
+      k = a < 3 ? (o < k ? o : k) : (o > k ? o : k);
+    }
+    part[6 * (size_t)blockIdx.x + a] = k;
+  }
+}
+
+// One block folds the per-block centroid keys into cbounds[6].
+__global__ __launch_bounds__(256) void k_centroid_box(const uint32_t* __restrict__ part, int parts,
+                                                      uint32_t* __restrict__ cbounds) {
+  __shared__ uint32_t wave_keys[4][6];
+  uint32_t k[6] = {0xffffffffu, 0xffffffffu, 0xffffffffu, 0u, 0u, 0u};
+  for (int j = threadIdx.x; j < parts; j += blockDim.x)
+    for (int a = 0; a < 6; ++a) {
+      const uint32_t o = part[6 * (size_t)j + a];
+      k[a] = a < 3 ? (o < k[a] ? o : k[a]) : (o > k[a] ? o : k[a]);
+    }
+  for (int a = 0; a < 6; ++a)
+    for (int m = 32; m >= 1; m >>= 1) {
+      const uint32_t o = __shfl_xor(k[a], m);
+      k[a] = a < 3 ? (o < k[a] ? o : k[a]) : (o > k[a] ? o : k[a]);
+    }
+  const int wave = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0)
+    for (int a = 0; a < 6; ++a) wave_keys[wave][a] = k[a];
+  __syncthreads();
+  if (threadIdx.x < 6) {
+    const int a = threadIdx.x;
+    uint32_t r = wave_keys[0][a];
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) {
+      const uint32_t o = wave_keys[w][a];
+      r = a < 3 ? (o < r ? o : r) : (o > r ? o : r);
+    }
+    cbounds[a] = r;
   }
 }
 
@@ -191,34 +236,58 @@ __global__ void k_karras(const uint32_t* __restrict__ codes, int n, int32_t* __r
   if (i == 0) parent[0] = -1;
 }
 
-__global__ void k_refit(const uint32_t* __restrict__ ids, const float* __restrict__ tb, int n,
-                        const int32_t* __restrict__ left, const int32_t* __restrict__ right,
-                        const int32_t* __restrict__ parent, float* bounds, uint32_t* flags) {
+// Bottom-up refit without inter-wave signalling.  Release/acquire fences at
+// device scope write back and invalidate the XCD-local L2 on this part, so
+// the refit orders itself with launch boundaries instead: internal node
+// depths come from pointer jumping over the parent array, then one launch
+// per depth, deepest first, unions the two child boxes of that depth's nodes.
+__global__ void k_leaf_bounds(const uint32_t* __restrict__ ids, const float* __restrict__ tb, int n,
+                              float* __restrict__ bounds) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
-  const int leaf = n - 1 + k;
+  const size_t leaf = (size_t)(n - 1 + k);
   const uint32_t id = ids[k];
-  for (int a = 0; a < 6; ++a) bounds[6 * (size_t)leaf + a] = tb[6 * (size_t)id + a];
-  if (n == 1) return;
-  __threadfence();
-  int p = parent[leaf];
-  // a radix tree over 30-bit codes tie-broken by 27-bit indices is at most 58
-  // levels deep; the cap only guards against a corrupted parent array
-  for (int guard = 0; p >= 0 && p < n - 1 && guard < 128; ++guard) {
-    const uint32_t old = atomicAdd(&flags[p], 1u);
-    if (old == 0) return;
-    __threadfence();
-    const volatile float* vb = bounds;
-    const int l = left[p], r = right[p];
-    float b[6];
-    for (int a = 0; a < 3; ++a) {
-      b[a] = dmin(vb[6 * (size_t)l + a], vb[6 * (size_t)r + a]);
-      b[3 + a] = dmax(vb[6 * (size_t)l + 3 + a], vb[6 * (size_t)r + 3 + a]);
-    }
-    for (int a = 0; a < 6; ++a) bounds[6 * (size_t)p + a] = b[a];
-    __threadfence();
-    p = parent[p];
+  for (int a = 0; a < 6; ++a) bounds[6 * leaf + a] = tb[6 * (size_t)id + a];
+}
+
+// up[i] = parent of internal node i (-1 at the root), dist[i] = 1 (0 at the root).
+__global__ void k_depth_init(const int32_t* __restrict__ parent, int m, int32_t* __restrict__ up,
+                             int32_t* __restrict__ dist) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  const int32_t p = parent[i];
+  const bool valid = p >= 0 && p < m;
+  up[i] = valid ? p : -1;
+  dist[i] = valid ? 1 : 0;
+}
+
+// One pointer-jumping step: afterwards up[i] is twice as far above i, and
+// dist[i] counts the edges from i to it.  Once up[i] is -1, dist[i] is the depth.
+__global__ void k_depth_jump(const int32_t* __restrict__ up_in, const int32_t* __restrict__ dist_in, int m,
+                             int32_t* __restrict__ up_out, int32_t* __restrict__ dist_out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  const int32_t u = up_in[i];
+  if (u < 0) {
+    up_out[i] = -1;
+    dist_out[i] = dist_in[i];
+  } else {
+    up_out[i] = up_in[u];
+    dist_out[i] = dist_in[i] + dist_in[u];
   }
+}
+
+__global__ void k_refit_level(const int32_t* __restrict__ depth, int level, int m, const int32_t* __restrict__ left,
+                              const int32_t* __restrict__ right, float* __restrict__ bounds) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m || depth[i] != level) return;
+  const size_t l = (size_t)left[i], r = (size_t)right[i];
+  float b[6];
+  for (int a = 0; a < 3; ++a) {
+    b[a] = dmin(bounds[6 * l + a], bounds[6 * r + a]);
+    b[3 + a] = dmax(bounds[6 * l + 3 + a], bounds[6 * r + 3 + a]);
+  }
+  for (int a = 0; a < 6; ++a) bounds[6 * (size_t)i + a] = b[a];
 }
 
 __global__ void k_collapse_count(const int32_t* __restrict__ srcs, int count, BinaryTreeView t,
@@ -394,13 +463,13 @@ bool build_device_scene(const float* d_pos, const float* d_col, const uint32_t* 
   uint32_t* hist = nullptr;
   int32_t *left = nullptr, *right = nullptr, *parent = nullptr, *range = nullptr;
   float* bounds = nullptr;
-  uint32_t* flags = nullptr;
+  uint32_t* cpart = nullptr;
+  int32_t* dist = nullptr;
   int32_t *srcs_a = nullptr, *srcs_b = nullptr, *depth = nullptr, *stack_bound = nullptr;
   uint32_t *icount = nullptr, *ioffs = nullptr;
   ScanScratch scan;
   bool ok = false;
   const int max_nodes = n > 1 ? n - 1 : 1;
-  std::vector<uint32_t> init_bounds = {0xffffffffu, 0xffffffffu, 0xffffffffu, 0u, 0u, 0u};
   int node_total = 0, level_count = 0, host_sb = 0;
   do {
 #define LB_STEP(x)                                                                         \
@@ -426,7 +495,8 @@ bool build_device_scene(const float* d_pos, const float* d_col, const uint32_t* 
       LB_STEP(hipMalloc(&parent, sizeof(int32_t) * (size_t)nb));
       LB_STEP(hipMalloc(&range, sizeof(int32_t) * 2 * (size_t)nb));
       LB_STEP(hipMalloc(&bounds, sizeof(float) * 6 * (size_t)nb));
-      LB_STEP(hipMalloc(&flags, sizeof(uint32_t) * (size_t)max_nodes));
+      LB_STEP(hipMalloc(&cpart, sizeof(uint32_t) * 6 * (size_t)blocks_for(n, 256)));
+      LB_STEP(hipMalloc(&dist, sizeof(int32_t) * (size_t)max_nodes));
       LB_STEP(hipMalloc(&srcs_a, sizeof(int32_t) * (size_t)max_nodes));
       LB_STEP(hipMalloc(&srcs_b, sizeof(int32_t) * (size_t)max_nodes));
       LB_STEP(hipMalloc(&depth, sizeof(int32_t) * (size_t)max_nodes));
@@ -447,11 +517,10 @@ bool build_device_scene(const float* d_pos, const float* d_col, const uint32_t* 
     if (fail) break;
     fail = true;
     do {
-      LB_STEP(hipMemcpyAsync(cbounds, init_bounds.data(), sizeof(uint32_t) * 6, hipMemcpyHostToDevice, stream));
-      LB_STEP(hipMemsetAsync(flags, 0, sizeof(uint32_t) * (size_t)max_nodes, stream));
       LB_STEP(hipMemsetAsync(stack_bound, 0, sizeof(int32_t), stream));
       LB_STEP(hipMemsetAsync(depth, 0, sizeof(int32_t), stream));
-      hipLaunchKernelGGL(k_tri_bounds, dim3(blocks_for(n, 256)), dim3(256), 0, stream, d_pos, n, tb, cen, cbounds);
+      hipLaunchKernelGGL(k_tri_bounds, dim3(blocks_for(n, 256)), dim3(256), 0, stream, d_pos, n, tb, cen, cpart);
+      hipLaunchKernelGGL(k_centroid_box, dim3(1), dim3(256), 0, stream, cpart, blocks_for(n, 256), cbounds);
       hipLaunchKernelGGL(k_morton, dim3(blocks_for(n, 256)), dim3(256), 0, stream, cen, n, cbounds, codes_a, ids_a);
       // 30-bit keys: passes at bits 0, 8, 16, 24
       uint32_t *kin = codes_a, *kout = codes_b, *vin = ids_a, *vout = ids_b;
@@ -470,8 +539,28 @@ bool build_device_scene(const float* d_pos, const float* d_col, const uint32_t* 
       LB_STEP(hipMemcpyAsync(out->sorted_ids, vin, sizeof(uint32_t) * (size_t)n, hipMemcpyDeviceToDevice, stream));
       hipLaunchKernelGGL(k_karras, dim3(blocks_for(n, 256)), dim3(256), 0, stream, kin, n, left, right, parent, range);
       if (n == 1) LB_STEP(hipMemsetAsync(parent, 0xff, sizeof(int32_t), stream));
-      hipLaunchKernelGGL(k_refit, dim3(blocks_for(n, 256)), dim3(256), 0, stream, vin, tb, n, left, right, parent,
-                         bounds, flags);
+      hipLaunchKernelGGL(k_leaf_bounds, dim3(blocks_for(n, 256)), dim3(256), 0, stream, vin, tb, n, bounds);
+      if (n > 1) {
+        // Split prefixes grow strictly downwards and take at most
+        // 30 + ceil(log2 n) values (30-bit codes, then index bits), which
+        // bounds the depth of internal nodes; n < 2^27 keeps it below 64, so
+        // six pointer-jumping steps settle every depth (seven are run).
+        const int m = n - 1;
+        int log2n = 0;
+        while ((1 << log2n) < n) ++log2n;
+        const int max_level = 30 + log2n;
+        int32_t *up_a = srcs_a, *up_b = srcs_b, *dist_a = dist, *dist_b = (int32_t*)icount;
+        hipLaunchKernelGGL(k_depth_init, dim3(blocks_for(m, 256)), dim3(256), 0, stream, parent, m, up_a, dist_a);
+        for (int step = 0; step < 7; ++step) {
+          hipLaunchKernelGGL(k_depth_jump, dim3(blocks_for(m, 256)), dim3(256), 0, stream, up_a, dist_a, m, up_b,
+                             dist_b);
+          int32_t* t = up_a; up_a = up_b; up_b = t;
+          t = dist_a; dist_a = dist_b; dist_b = t;
+        }
+        for (int level = max_level; level >= 0; --level)
+          hipLaunchKernelGGL(k_refit_level, dim3(blocks_for(m, 256)), dim3(256), 0, stream, dist_a, level, m, left,
+                             right, bounds);
+      }
       LB_STEP(hipGetLastError());
       BinaryTreeView tv;
       tv.left = left; tv.right = right; tv.bounds = bounds; tv.range = range; tv.n = n;
@@ -524,7 +613,7 @@ bool build_device_scene(const float* d_pos, const float* d_col, const uint32_t* 
   } while (0);
   for (void* p : {(void*)tb, (void*)cen, (void*)cbounds, (void*)codes_a, (void*)codes_b, (void*)ids_a,
                   (void*)ids_b, (void*)hist, (void*)left, (void*)right, (void*)parent, (void*)range,
-                  (void*)bounds, (void*)flags, (void*)srcs_a, (void*)srcs_b, (void*)depth, (void*)stack_bound,
+                  (void*)bounds, (void*)cpart, (void*)dist, (void*)srcs_a, (void*)srcs_b, (void*)depth, (void*)stack_bound,
                   (void*)icount, (void*)ioffs})
     if (p) (void)hipFree(p);
   scan_release(&scan);
