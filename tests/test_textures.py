@@ -13,7 +13,7 @@ import oracle
 from compound_ray_amd import eyerenderer as er
 from compound_ray_amd import scenes
 
-REF_SCENE = pathlib.Path("/root/reference/data/natural-standin-sky.gltf")
+import standin
 
 
 # ------------------------------------------------------------ PNG writer ----
@@ -240,11 +240,11 @@ def python_png_rgba(data: bytes) -> np.ndarray:
     return img
 
 
-@pytest.mark.skipif(not REF_SCENE.exists(), reason="reference data not present")
-def test_reference_scene_texture(lib):
+def test_reference_scene_texture(lib, tmp_path):
     """The reference's textured scene (natural-standin-sky.gltf): its embedded
     PNG decodes to the same texels as an independent decoder, and its sampler
     (LINEAR / LINEAR_MIPMAP_LINEAR, no wrap given) maps to repeat + linear."""
+    REF_SCENE = standin.unpack(tmp_path)["gltf"]
     desc, texels = er.parse_textures_np(lib, str(REF_SCENE))
     doc = json.loads(REF_SCENE.read_text())
     bv = doc["bufferViews"][doc["images"][0]["bufferView"]]
