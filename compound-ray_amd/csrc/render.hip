@@ -13,6 +13,7 @@ namespace eye {
 namespace {
 
 constexpr int kTraceThreads = 256;
+constexpr int kPacketStackMax = 128;  // two VGPRs of 64 lanes
 
 struct SceneView {
   const WideNode* nodes;
@@ -84,6 +85,49 @@ __device__ __forceinline__ void test_leaf(P4 tris, int32_t code, const RayPre& r
         h.b2 = b2;
         h.k = k;
         h.id = id;
+      }
+    }
+  }
+}
+
+__device__ __forceinline__ float lane_f(int v, int l) { return __int_as_float(__builtin_amdgcn_readlane(v, l)); }
+
+// A wave-uniform leaf (packet traversal, every lane of the wave executing).
+// Triangle records are 12 dwords (three float4: vertex xyz, and the original
+// id in the first record's w).  One coalesced load puts dword j of up to five
+// consecutive records on lane j; v_readlane then hands each record to the
+// scalar unit.  A leaf of n triangles costs ceil(n / 5) dependent memory round
+// trips instead of n, and the tested values are the same bits.
+__device__ __forceinline__ void test_leaf_wave(const float4* tris, int32_t code, const RayPre& rp, Hit& h) {
+  constexpr uint32_t kPerLoad = 5;
+  const uint32_t first = leaf_first(code);
+  const uint32_t cnt = leaf_count(code);
+  const int lane = (int)(threadIdx.x & 63);
+  const int* words = reinterpret_cast<const int*>(tris + 3 * (size_t)first);
+  for (uint32_t q0 = 0; q0 < cnt; q0 += kPerLoad) {
+    const uint32_t m = cnt - q0 < kPerLoad ? cnt - q0 : kPerLoad;
+    const int v = lane < (int)(12 * m) ? words[12 * (size_t)q0 + lane] : 0;
+    for (uint32_t q = 0; q < m; ++q) {
+      const int o = (int)(12 * q);
+      const V3 a = v3(__int_as_float(__builtin_amdgcn_readlane(v, o + 0)),
+                      __int_as_float(__builtin_amdgcn_readlane(v, o + 1)),
+                      __int_as_float(__builtin_amdgcn_readlane(v, o + 2)));
+      const uint32_t id = (uint32_t)__builtin_amdgcn_readlane(v, o + 3);
+      const V3 b = v3(__int_as_float(__builtin_amdgcn_readlane(v, o + 4)),
+                      __int_as_float(__builtin_amdgcn_readlane(v, o + 5)),
+                      __int_as_float(__builtin_amdgcn_readlane(v, o + 6)));
+      const V3 c = v3(__int_as_float(__builtin_amdgcn_readlane(v, o + 8)),
+                      __int_as_float(__builtin_amdgcn_readlane(v, o + 9)),
+                      __int_as_float(__builtin_amdgcn_readlane(v, o + 10)));
+      float t, b1, b2;
+      if (intersect_triangle(rp, a, b, c, &t, &b1, &b2)) {
+        if (t < h.t || (t == h.t && id < h.id)) {
+          h.t = t;
+          h.b1 = b1;
+          h.b2 = b2;
+          h.k = first + q0 + q;
+          h.id = id;
+        }
       }
     }
   }
@@ -211,103 +255,128 @@ __device__ __forceinline__ void locate_ray(const EyeLaunch& L, uint64_t r, int* 
 // ---------------------------------------------------------------------------
 // Wave-packet traversal.  The 64 rays of a wave are samples of one or two
 // neighbouring ommatidia: nearly the same origin and direction.  The wave
-// walks the BVH together: the node index is wave-uniform (scalar loads, one
-// 64-entry stack per wave in LDS), each lane tests the children against its
-// own ray and closest hit, a child is entered when any lane's ray enters it,
-// and every active lane tests the triangles of an entered leaf.  Each lane
-// still computes the exact lexicographic (t, id) minimum over the triangles
-// its ray can hit (a lane only ever visits a superset of the nodes its own
-// traversal would), so results are identical to per-ray traversal.
+// walks the BVH together: the node index is wave-uniform (scalar loads), each
+// lane tests the children against its own ray and closest hit, a child is
+// entered when any lane's ray enters it, and every active lane tests the
+// triangles of an entered leaf.  Each lane still computes the exact
+// lexicographic (t, id) minimum over the triangles its ray can hit (a lane
+// only ever visits a superset of the nodes its own traversal would), so
+// results are identical to per-ray traversal whatever the visiting order.
+//
+// A step is one long chain of dependent instructions, so the step keeps off
+// LDS: the wave's stack lives in two VGPRs, entry k in lane k & 63 of
+// stack[k >> 6] (a lane select to push, v_readlane to pop; kPacketStackMax
+// entries), and entered children are ordered nearest first by the entry
+// distance of the first lane that enters them, read with v_readlane and
+// compared as bits on the scalar unit (entry distances are >= tmin >= 0,
+// where the float order is the integer order).
 // ---------------------------------------------------------------------------
+__device__ __forceinline__ void stack_push(int& s0, int& s1, int lane, int sp, int v) {
+  if (sp < 64)
+    s0 = lane == sp ? v : s0;
+  else
+    s1 = lane == sp - 64 ? v : s1;
+}
+__device__ __forceinline__ int stack_top(int s0, int s1, int sp) {
+  return sp < 64 ? __builtin_amdgcn_readlane(s0, sp) : __builtin_amdgcn_readlane(s1, sp - 64);
+}
+
+// One node of a packet step: tests its leaf children (every lane, against
+// its own ray) and appends the internal children that any lane enters to
+// ic / ik (child, ordering key).
 template <bool kParallel, bool kScalar>
-__device__ __forceinline__ void closest_hit_packet_impl(const SceneView& S, const RayPre& rp, bool active,
-                                                        int* wstack, Hit& h) {
+__device__ __forceinline__ void packet_visit(const SceneView& S, const RayPre& rp, bool active, int node, Hit& h,
+                                             int* ic, uint32_t* ik, int& ni) {
+  float4 lx, hx, ly, hy, lz, hz;
+  int4 ch;
+  if (kScalar) {
+    const UniformPtr np{S.nodes + node};
+    lx = load4(np, 0); hx = load4(np, 1); ly = load4(np, 2);
+    hy = load4(np, 3); lz = load4(np, 4); hz = load4(np, 5);
+    ch = load_int4_uniform(np.p, 6);
+  } else {
+    const float4* np = reinterpret_cast<const float4*>(S.nodes + node);
+    lx = np[0]; hx = np[1]; ly = np[2]; hy = np[3]; lz = np[4]; hz = np[5];
+    ch = reinterpret_cast<const int4*>(np)[6];
+  }
+  const float limit = h.t * kCullSlack;
+  float te[4];
+  te[0] = slab_enter<kParallel>(rp, lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, limit);
+  te[1] = slab_enter<kParallel>(rp, lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, limit);
+  te[2] = slab_enter<kParallel>(rp, lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, limit);
+  te[3] = slab_enter<kParallel>(rp, lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, limit);
+  const int32_t cc[4] = {ch.x, ch.y, ch.z, ch.w};
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    const int32_t c = cc[s];
+    const bool mine = active && te[s] < INFINITY;
+    if (c == kChildEmpty || __ballot(mine) == 0ull) continue;
+    if (child_is_leaf(c)) {
+      // inactive lanes test a dummy ray whose result is discarded
+      if (kScalar)
+        test_leaf(UniformPtr{S.tris}, c, rp, h);
+      else
+        test_leaf_wave(S.tris, c, rp, h);
+    } else {
+      // order key: the entry distance of the first lane that enters the child
+      const uint64_t m = __ballot(mine);
+      const int lead = __builtin_amdgcn_readfirstlane(__ffsll((long long)m) - 1);
+      ic[ni] = c;
+      ik[ni] = (uint32_t)__builtin_amdgcn_readlane((int)__float_as_uint(te[s]), lead);
+      ++ni;
+    }
+  }
+}
+
+template <bool kParallel, bool kScalar>
+__device__ __forceinline__ void closest_hit_packet_impl(const SceneView& S, const RayPre& rp, bool active, Hit& h) {
   int sp = 0;
   int node = 0;
+  int stk0 = 0, stk1 = 0;
+  const int lane = (int)(threadIdx.x & 63);
   for (;;) {
     node = __builtin_amdgcn_readfirstlane(node);
-    float4 lx, hx, ly, hy, lz, hz;
-    int4 ch;
-    if (kScalar) {
-      const UniformPtr np{S.nodes + node};
-      lx = load4(np, 0); hx = load4(np, 1); ly = load4(np, 2);
-      hy = load4(np, 3); lz = load4(np, 4); hz = load4(np, 5);
-      ch = load_int4_uniform(np.p, 6);
-    } else {
-      const float4* np = reinterpret_cast<const float4*>(S.nodes + node);
-      lx = np[0]; hx = np[1]; ly = np[2]; hy = np[3]; lz = np[4]; hz = np[5];
-      ch = reinterpret_cast<const int4*>(np)[6];
-    }
-    const float limit = h.t * kCullSlack;
-    float te[4];
-    te[0] = slab_enter<kParallel>(rp, lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, limit);
-    te[1] = slab_enter<kParallel>(rp, lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, limit);
-    te[2] = slab_enter<kParallel>(rp, lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, limit);
-    te[3] = slab_enter<kParallel>(rp, lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, limit);
-    const int32_t cc[4] = {ch.x, ch.y, ch.z, ch.w};
     int ni = 0;
     int ic[4];
-    float it[4];
-#pragma unroll
-    for (int s = 0; s < 4; ++s) {
-      const int32_t c = cc[s];
-      const bool mine = active && te[s] < INFINITY;
-      if (c == kChildEmpty || __ballot(mine) == 0ull) continue;
-      if (child_is_leaf(c)) {
-        // uniform triangle loads; inactive lanes test a dummy ray whose result is discarded
-        if (kScalar)
-          test_leaf(UniformPtr{S.tris}, c, rp, h);
-        else
-          test_leaf(VaryingPtr{S.tris}, c, rp, h);
-      } else {
-        // order key: the entry distance of the first lane that enters the child
-        const uint64_t m = __ballot(mine);
-        const int lead = __ffsll((long long)m) - 1;
-        ic[ni] = c;
-        it[ni] = __shfl(te[s], lead);
-        ++ni;
-      }
-    }
+    uint32_t ik[4];
+    packet_visit<kParallel, kScalar>(S, rp, active, node, h, ic, ik, ni);
     if (ni == 0) {
       if (sp == 0) break;
       --sp;
-      node = wstack[sp];
+      node = stack_top(stk0, stk1, sp);
       continue;
     }
     for (int a = 1; a < ni; ++a) {
-      const float tv = it[a];
+      const uint32_t kv = ik[a];
       const int cv = ic[a];
       int b = a - 1;
-      while (b >= 0 && it[b] > tv) {
-        it[b + 1] = it[b];
+      while (b >= 0 && ik[b] > kv) {
+        ik[b + 1] = ik[b];
         ic[b + 1] = ic[b];
         --b;
       }
-      it[b + 1] = tv;
+      ik[b + 1] = kv;
       ic[b + 1] = cv;
     }
-    // every lane holds the same children: lane 0 writes the wave's stack
-    if ((threadIdx.x & 63) == 0)
-      for (int a = ni - 1; a >= 1; --a) wstack[sp + (ni - 1 - a)] = ic[a];
-    sp += ni - 1;
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
+    for (int a = ni - 1; a >= 1; --a) {
+      stack_push(stk0, stk1, lane, sp, ic[a]);
+      ++sp;
+    }
     node = ic[0];
   }
 }
 
 template <bool kScalar = false>
-__device__ __forceinline__ void closest_hit_packet(const SceneView& S, const RayPre& rp, bool active, int* wstack,
-                                                   Hit& h) {
+__device__ __forceinline__ void closest_hit_packet(const SceneView& S, const RayPre& rp, bool active, Hit& h) {
   h.t = rp.tmax;
   h.b1 = h.b2 = 0.0f;
   h.k = 0xffffffffu;
   h.id = 0xffffffffu;
   if (S.node_count == 0) return;
   if (__ballot(active && rp.par != 0) != 0ull)
-    closest_hit_packet_impl<true, kScalar>(S, rp, active, wstack, h);
+    closest_hit_packet_impl<true, kScalar>(S, rp, active, h);
   else
-    closest_hit_packet_impl<false, kScalar>(S, rp, active, wstack, h);
+    closest_hit_packet_impl<false, kScalar>(S, rp, active, h);
 }
 
 __device__ __forceinline__ float wave_sum(float v) {
@@ -355,13 +424,12 @@ __device__ __forceinline__ int pow2_at_least(uint32_t n) {
 // wave stay busy when sample counts are small (adaptive pilots, uneven
 // counts).  Each unit's colour is the same canonical sum either way.
 template <bool kStats, bool kSamples, int kTrav>
-__global__ __launch_bounds__(kTraceThreads) void k_render_units(SceneView S, UnitLaunch U, int wstride) {
-  // LDS: one stack per wave (packets) or one column per lane (per-ray,
-  // laid out [depth][blockDim.x])
-  extern __shared__ int wave_stacks[];
+__global__ __launch_bounds__(kTraceThreads) void k_render_units(SceneView S, UnitLaunch U) {
+  // LDS (per-ray traversal only): one stack column per lane, laid out
+  // [depth][blockDim.x]; packets keep their stack in VGPR lanes
+  extern __shared__ uint32_t unit_lds[];
   const int lane = threadIdx.x & 63;
-  int* wst = wave_stacks + (threadIdx.x >> 6) * wstride;
-  uint32_t* lane_stack = reinterpret_cast<uint32_t*>(wave_stacks) + threadIdx.x;
+  uint32_t* lane_stack = unit_lds + threadIdx.x;
   const uint64_t units = (uint64_t)U.pose_count * (uint64_t)U.omm_count;
   for (;;) {
     uint32_t base = 0;
@@ -437,7 +505,7 @@ __global__ __launch_bounds__(kTraceThreads) void k_render_units(SceneView S, Uni
           h.b1 = h.b2 = 0.0f;
           if (active) closest_hit(S, rp, lane_stack, (int)blockDim.x, h);
         } else {
-          closest_hit_packet<kTrav == kUnitPacketScalar>(S, rp, active, wst, h);
+          closest_hit_packet<kTrav == kUnitPacketScalar>(S, rp, active, h);
         }
         if (active) {
           const V3 c = shade(S, h, ray.dx, ray.dy, ray.dz);
@@ -612,10 +680,8 @@ __global__ __launch_bounds__(kTraceThreads, 5) void k_render_chunks(SceneView S,
 // completion counters and last-wave reduction as k_render_chunks (no
 // statistics).
 template <bool kSamples, bool kScalar>
-__global__ __launch_bounds__(kTraceThreads, 6) void k_render_tasks(SceneView S, UnitLaunch U, int wstride, int chunks) {
-  extern __shared__ int wave_stacks[];
+__global__ __launch_bounds__(kTraceThreads, 6) void k_render_tasks(SceneView S, UnitLaunch U, int chunks) {
   const int lane = threadIdx.x & 63;
-  int* wst = wave_stacks + (threadIdx.x >> 6) * wstride;
   const uint64_t units = (uint64_t)U.pose_count * (uint64_t)U.omm_count;
   const uint64_t tasks = units * (uint64_t)chunks;
   const uint64_t t = (uint64_t)blockIdx.x * (kTraceThreads / 64) + (uint64_t)(threadIdx.x >> 6);
@@ -643,7 +709,7 @@ __global__ __launch_bounds__(kTraceThreads, 6) void k_render_tasks(SceneView S, 
   }
   const RayPre rp = precompute_ray(ray);
   Hit h;
-  closest_hit_packet<kScalar>(S, rp, active, wst, h);
+  closest_hit_packet<kScalar>(S, rp, active, h);
   float4 a = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   if (active) {
     const V3 c = shade(S, h, ray.dx, ray.dy, ray.dz);
@@ -803,17 +869,17 @@ static const char* g_last_kernel = "";
 static bool g_split = true;
 
 template <int kTrav>
-static void launch_units_variant(const UnitLaunch& U, const SceneView& v, int blocks, size_t lds, int wstride,
+static void launch_units_variant(const UnitLaunch& U, const SceneView& v, int blocks, size_t lds,
                                  hipStream_t stream) {
   const dim3 grid(blocks), block(kTraceThreads);
   if (U.stats && U.samples)
-    hipLaunchKernelGGL((k_render_units<true, true, kTrav>), grid, block, lds, stream, v, U, wstride);
+    hipLaunchKernelGGL((k_render_units<true, true, kTrav>), grid, block, lds, stream, v, U);
   else if (U.stats)
-    hipLaunchKernelGGL((k_render_units<true, false, kTrav>), grid, block, lds, stream, v, U, wstride);
+    hipLaunchKernelGGL((k_render_units<true, false, kTrav>), grid, block, lds, stream, v, U);
   else if (U.samples)
-    hipLaunchKernelGGL((k_render_units<false, true, kTrav>), grid, block, lds, stream, v, U, wstride);
+    hipLaunchKernelGGL((k_render_units<false, true, kTrav>), grid, block, lds, stream, v, U);
   else
-    hipLaunchKernelGGL((k_render_units<false, false, kTrav>), grid, block, lds, stream, v, U, wstride);
+    hipLaunchKernelGGL((k_render_units<false, false, kTrav>), grid, block, lds, stream, v, U);
 }
 
 static void launch_chunks(const UnitLaunch& U, const SceneView& v, int blocks, size_t lds, int chunks,
@@ -849,9 +915,13 @@ static int device_cus() {
 }
 
 static size_t units_lds_bytes(const DeviceScene& scene, int traversal) {
-  if (traversal == kUnitPerRay) return stack_bytes(scene);
-  const int wstride = scene.stack_bound > 0 ? scene.stack_bound : 1;
-  return sizeof(int) * (size_t)wstride * (kTraceThreads / 64);
+  return traversal == kUnitPerRay ? stack_bytes(scene) : 0;
+}
+
+// Packets keep their traversal stack in VGPR lanes (kPacketStackMax entries);
+// a deeper BVH takes the per-ray traversal, whose stacks live in LDS.
+static int effective_traversal(const DeviceScene& scene, int traversal) {
+  return traversal != kUnitPerRay && scene.stack_bound > kPacketStackMax ? (int)kUnitPerRay : traversal;
 }
 
 static const void* units_kernel_of(const UnitLaunch& U) {
@@ -860,7 +930,9 @@ static const void* units_kernel_of(const UnitLaunch& U) {
   return units_kernel_variant<kUnitPacket>(U);
 }
 
-int render_units_capacity(const DeviceScene& scene, const UnitLaunch& U) {
+int render_units_capacity(const DeviceScene& scene, const UnitLaunch& U_in) {
+  UnitLaunch U = U_in;
+  U.traversal = effective_traversal(scene, U.traversal);
   const void* fn = units_kernel_of(U);
   int per_cu = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kTraceThreads, units_lds_bytes(scene, U.traversal)) !=
@@ -870,7 +942,8 @@ int render_units_capacity(const DeviceScene& scene, const UnitLaunch& U) {
   return device_cus() * per_cu;
 }
 
-bool launch_render_units(const DeviceScene& scene, const UnitLaunch& U, hipStream_t stream, std::string* err) {
+bool launch_render_units(const DeviceScene& scene, const UnitLaunch& U_in, hipStream_t stream, std::string* err) {
+  UnitLaunch U = U_in;
   const uint64_t units = (uint64_t)U.pose_count * (uint64_t)U.omm_count;
   if (units == 0) return true;
   if (units > 0xffffffffull - 65536ull) {
@@ -885,7 +958,7 @@ bool launch_render_units(const DeviceScene& scene, const UnitLaunch& U, hipStrea
     if (err) *err = "render_units: unknown traversal";
     return false;
   }
-  const int wstride = scene.stack_bound > 0 ? scene.stack_bound : 1;
+  U.traversal = effective_traversal(scene, U.traversal);
   const size_t lds = units_lds_bytes(scene, U.traversal);
   if (lds > 160 * 1024) {
     if (err) *err = "render_units: BVH too deep for the LDS traversal stack";
@@ -910,13 +983,13 @@ bool launch_render_units(const DeviceScene& scene, const UnitLaunch& U, hipStrea
       const dim3 grid((unsigned)((tasks + (kTraceThreads / 64) - 1) / (kTraceThreads / 64))), block(kTraceThreads);
       const bool scalar = U.traversal == kUnitPacketScalar;
       if (U.samples && scalar)
-        hipLaunchKernelGGL((k_render_tasks<true, true>), grid, block, lds, stream, v, U, wstride, (int)chunks);
+        hipLaunchKernelGGL((k_render_tasks<true, true>), grid, block, 0, stream, v, U, (int)chunks);
       else if (U.samples)
-        hipLaunchKernelGGL((k_render_tasks<true, false>), grid, block, lds, stream, v, U, wstride, (int)chunks);
+        hipLaunchKernelGGL((k_render_tasks<true, false>), grid, block, 0, stream, v, U, (int)chunks);
       else if (scalar)
-        hipLaunchKernelGGL((k_render_tasks<false, true>), grid, block, lds, stream, v, U, wstride, (int)chunks);
+        hipLaunchKernelGGL((k_render_tasks<false, true>), grid, block, 0, stream, v, U, (int)chunks);
       else
-        hipLaunchKernelGGL((k_render_tasks<false, false>), grid, block, lds, stream, v, U, wstride, (int)chunks);
+        hipLaunchKernelGGL((k_render_tasks<false, false>), grid, block, 0, stream, v, U, (int)chunks);
       g_last_kernel = "k_render_tasks";
       return check_launch("render_tasks", err);
     }
@@ -955,11 +1028,11 @@ bool launch_render_units(const DeviceScene& scene, const UnitLaunch& U, hipStrea
   if (waves > need) blocks = (int)((need + (kTraceThreads / 64) - 1) / (kTraceThreads / 64));
   hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, stream, G.queue, 1u, 0u);
   if (G.traversal == kUnitPerRay)
-    launch_units_variant<kUnitPerRay>(G, v, blocks, lds, wstride, stream);
+    launch_units_variant<kUnitPerRay>(G, v, blocks, lds, stream);
   else if (G.traversal == kUnitPacketScalar)
-    launch_units_variant<kUnitPacketScalar>(G, v, blocks, lds, wstride, stream);
+    launch_units_variant<kUnitPacketScalar>(G, v, blocks, lds, stream);
   else
-    launch_units_variant<kUnitPacket>(G, v, blocks, lds, wstride, stream);
+    launch_units_variant<kUnitPacket>(G, v, blocks, lds, stream);
   g_last_kernel = "k_render_units";
   return check_launch("render_units", err);
 }
