@@ -41,6 +41,14 @@ __global__ void k_features(const Ommatidium* __restrict__ omms, const Pose* __re
   f[11] = sqrtf(s1.y);
 }
 
+__device__ inline uint32_t lane_footprint(uint32_t e) {
+  if (e == 0u) return 0u;
+  if (e > 32u) return (e + 63u) & ~63u;
+  uint32_t p = 1u;
+  while (p < e) p <<= 1;
+  return p;
+}
+
 __device__ inline float required_samples(float log2_nreq) {
   return exp2f(fminf(fmaxf(log2_nreq, 0.0f), 20.0f));
 }
@@ -66,7 +74,11 @@ __global__ __launch_bounds__(kThreads) void k_allocate(const float* __restrict__
   for (int i = threadIdx.x; i < omm_count; i += kThreads) {
     const float want = sqrtf(required_samples(pred[base + i])) * S / (float)omm_count * scale;
     const float c = fminf(fmaxf(ceilf(want), (float)pilot), (float)max_spp);
-    const uint32_t n = (uint32_t)c;
+    // The top-up pass gives a unit of e samples a power-of-two lane segment
+    // (e <= 32) or whole 64-lane rounds (e > 32); those lanes cost the same
+    // time idle or busy, so the top-up is rounded up to fill them.
+    const uint32_t e = lane_footprint((uint32_t)c - (uint32_t)pilot);
+    const uint32_t n = min((uint32_t)pilot + e, (uint32_t)max_spp);
     counts[base + i] = n;
     extra[base + i] = n - (uint32_t)pilot;
     mine += n;

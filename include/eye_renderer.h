@@ -220,9 +220,10 @@ void setTraversalMode(int mode);
 /* Keep the per-sample records of renderFrame for getCurrentEyeSamples (off by
  * default: the packet renderer reduces samples in registers). */
 void setSampleRecording(bool enabled);
-/* Kernel of the last compound-eye render: "k_render_units", or
- * "k_render_chunks" for a split launch (uniform counts above 64 samples and
- * few units, where each unit's 64-sample chunks run on different waves). */
+/* Kernel of the last compound-eye render: "k_render_units", or for a split
+ * launch (uniform counts above 64 samples and few units, where each unit's
+ * 64-sample chunks run on different waves) "k_render_tasks" (wave packets)
+ * or "k_render_chunks" (per-ray traversal). */
 const char* getLastRenderKernel(void);
 /* Allows split launches (default true); results are the same either way. */
 void setSplitLaunches(bool enabled);
