@@ -160,7 +160,7 @@ bool AdaptiveRenderer::ensure(size_t poses, size_t units, std::string* err) {
     if (err) *err = "adaptive: event creation failed";
     return false;
   }
-  if (!d_total_ && (!grow(&d_total_, 1, err) || !grow(&d_queue_, 4, err))) return false;
+  if (!d_total_ && (!grow(&d_total_, 1, err) || !grow(&d_queue_, kQueueWords, err))) return false;
   if (poses > poses_cap_) {
     if (!grow(&d_poses_, poses, err)) return false;
     poses_cap_ = poses;

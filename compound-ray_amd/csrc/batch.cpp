@@ -176,7 +176,7 @@ bool BatchRenderer::prepare(const UnitLaunch& U, const std::vector<Pose>& poses,
   *realloc = poses_cap_ < np || colours_cap_ < units;
   if (!grow(&d_poses_, &poses_cap_, np, err)) return false;
   if (!grow(&d_colours_, &colours_cap_, units, err)) return false;
-  if (!d_queue_ && hipMalloc(&d_queue_, sizeof(uint32_t) * 4) != hipSuccess) {
+  if (!d_queue_ && hipMalloc(&d_queue_, sizeof(uint32_t) * kQueueWords) != hipSuccess) {
     if (err) *err = "batch: queue allocation failed";
     return false;
   }

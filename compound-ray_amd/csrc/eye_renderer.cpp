@@ -159,7 +159,7 @@ void ensure_hip() {
   hip_check(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking), "hipStreamCreate");
   hip_check(hipMalloc(&s.d_pose, sizeof(Pose)), "hipMalloc pose");
   hip_check(hipMalloc(&s.d_stats, sizeof(unsigned long long) * 4), "hipMalloc stats");
-  hip_check(hipMalloc(&s.d_queue, sizeof(uint32_t) * 4), "hipMalloc queue");
+  hip_check(hipMalloc(&s.d_queue, sizeof(uint32_t) * kQueueWords), "hipMalloc queue");
   for (hipEvent_t& e : s.ev) hip_check(hipEventCreate(&e), "hipEventCreate");
   s.hip_ready = true;
 }

@@ -48,7 +48,7 @@ struct UnitLaunch {
                                         //   (mean inverse depth, its variance, 0, 0)
   float4* samples = nullptr;            // nullable: per-sample (r, g, b, t or -1)
   const uint64_t* sample_offsets = nullptr;  // first sample record of each unit
-  uint32_t* queue = nullptr;            // device work counter
+  uint32_t* queue = nullptr;            // device work counters (kQueueWords)
   // Optional scratch for split launches (uniform sample counts above 64 and
   // few units): each unit's 64-sample chunks go to different waves; per-lane
   // chunk values (1 float4, 3 with stats) and a completion counter per unit.
@@ -67,6 +67,9 @@ struct UnitLaunch {
 // (a wave-uniform node, node records read with vector loads or through the
 // scalar unit), or every lane walks its own ray with an LDS stack per lane.
 enum UnitTraversal : int { kUnitPacket = 0, kUnitPacketScalar = 1, kUnitPerRay = 2 };
+
+// Work counters a launch needs at U.queue: one per slice of the unit range.
+constexpr int kQueueWords = 8;
 
 // Launches the unit renderer with U's launch shape.  The queue counter is
 // cleared by a kernel first.

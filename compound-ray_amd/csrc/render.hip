@@ -13,6 +13,9 @@ namespace eye {
 namespace {
 
 constexpr int kTraceThreads = 256;
+// XCDs of the device: workgroup b runs on XCD b % kXcds, each with its own L2.
+constexpr uint32_t kXcds = 8;
+static_assert(kQueueWords >= (int)kXcds, "one work counter per XCD slice");
 constexpr int kPacketStackMax = 128;  // two VGPRs of 64 lanes
 
 struct SceneView {
@@ -431,12 +434,30 @@ __global__ __launch_bounds__(kTraceThreads) void k_render_units(SceneView S, Uni
   const int lane = threadIdx.x & 63;
   uint32_t* lane_stack = unit_lds + threadIdx.x;
   const uint64_t units = (uint64_t)U.pose_count * (uint64_t)U.omm_count;
+  // The queue is split into kXcds slices of consecutive units with a counter
+  // each.  A wave takes units from its workgroup's XCD slice (neighbouring
+  // directions of the same poses share that XCD's L2) and, once the slice is
+  // empty, from the following slices; it exits when all are empty.
+  const uint64_t slice = (units + kXcds - 1) / kXcds;
+  uint32_t x = blockIdx.x % kXcds;
+  uint32_t tried = 0;
   for (;;) {
-    uint32_t base = 0;
-    if (lane == 0) base = atomicAdd(U.queue, (uint32_t)U.units_per_grab);
-    base = __builtin_amdgcn_readfirstlane(__shfl(base, 0));
-    if ((uint64_t)base >= units) break;
-    const int G = (int)min((uint64_t)U.units_per_grab, units - (uint64_t)base);
+    uint64_t base = 0, s_end = 0;
+    while (tried < kXcds) {
+      const uint64_t s0 = (uint64_t)x * slice, s1 = min(units, s0 + slice);
+      uint32_t off = 0xffffffffu;
+      if (lane == 0 && s0 < s1) off = atomicAdd(U.queue + x, (uint32_t)U.units_per_grab);
+      off = (uint32_t)__builtin_amdgcn_readfirstlane((int)__shfl(off, 0));
+      if (s0 < s1 && s0 + off < s1) {
+        base = s0 + off;
+        s_end = s1;
+        break;
+      }
+      x = (x + 1u) % kXcds;
+      ++tried;
+    }
+    if (tried >= kXcds) break;
+    const int G = (int)min((uint64_t)U.units_per_grab, s_end - base);
     // lane g < G holds the sample count of unit base + g
     uint32_t my_n = 0;
     if (lane < G) {
@@ -673,18 +694,35 @@ __global__ __launch_bounds__(kTraceThreads, 5) void k_render_chunks(SceneView S,
 }
 
 // Non-persistent form of the split launch for wave-packet traversal: wave w
-// of the grid is task w (chunk w % C of queue position w / C), the whole wave
-// returns together when w is past the last task, and nothing but the
+// of logical block L is task 4L + w (chunk of queue position task / C), the
+// whole wave returns together when past the last task, and nothing but the
 // traversal sits between the ray generation and the chunk store, which keeps
 // register use low (more resident waves).  Same per-lane chunk values,
 // completion counters and last-wave reduction as k_render_chunks (no
 // statistics).
+//
+// Workgroups are handed to the 8 XCDs round-robin (block b to XCD b % 8),
+// and each XCD has its own L2.  Logical blocks are therefore dealt to XCDs in
+// runs of kXcdRun consecutive blocks: the queue is in Morton order of the
+// ommatidial axes, so an XCD's resident waves look in neighbouring directions
+// and share BVH nodes in its L2, while runs still alternate between XCDs so
+// that each gets its share of costly directions.  The mapping is a
+// bijection, so results do not depend on it.
+constexpr uint32_t kXcdRun = 16;
+
+__device__ __forceinline__ uint32_t xcd_logical_block(uint32_t b, uint32_t blocks) {
+  const uint32_t span = kXcds * kXcdRun;
+  if (b >= (blocks / span) * span) return b;
+  const uint32_t x = b % kXcds, j = b / kXcds;
+  return ((j / kXcdRun) * kXcds + x) * kXcdRun + j % kXcdRun;
+}
 template <bool kSamples, bool kScalar>
 __global__ __launch_bounds__(kTraceThreads, 6) void k_render_tasks(SceneView S, UnitLaunch U, int chunks) {
   const int lane = threadIdx.x & 63;
   const uint64_t units = (uint64_t)U.pose_count * (uint64_t)U.omm_count;
   const uint64_t tasks = units * (uint64_t)chunks;
-  const uint64_t t = (uint64_t)blockIdx.x * (kTraceThreads / 64) + (uint64_t)(threadIdx.x >> 6);
+  const uint64_t t = (uint64_t)xcd_logical_block(blockIdx.x, gridDim.x) * (kTraceThreads / 64) +
+                     (uint64_t)(threadIdx.x >> 6);
   if (t >= tasks) return;  // whole wave
   const uint32_t n = (uint32_t)U.spp;
   const float inv = 1.0f / (float)n;
@@ -1011,7 +1049,7 @@ bool launch_render_units(const DeviceScene& scene, const UnitLaunch& U_in, hipSt
     }
     const uint64_t waves = (uint64_t)chunk_blocks * (kTraceThreads / 64);
     if (waves > tasks) chunk_blocks = (int)((tasks + (kTraceThreads / 64) - 1) / (kTraceThreads / 64));
-    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, stream, U.queue, 1u, 0u);
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, stream, U.queue, (uint32_t)kQueueWords, 0u);
     hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((units + 255) / 256)), dim3(256), 0, stream, U.chunk_done,
                        (uint32_t)units, 0u);
     launch_chunks(U, v, chunk_blocks, chunk_lds, (int)chunks, stream);
@@ -1026,7 +1064,7 @@ bool launch_render_units(const DeviceScene& scene, const UnitLaunch& U_in, hipSt
   const uint64_t need = (units + (uint64_t)G.units_per_grab - 1) / (uint64_t)G.units_per_grab;
   const uint64_t waves = (uint64_t)blocks * (kTraceThreads / 64);
   if (waves > need) blocks = (int)((need + (kTraceThreads / 64) - 1) / (kTraceThreads / 64));
-  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, stream, G.queue, 1u, 0u);
+  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, stream, G.queue, (uint32_t)kQueueWords, 0u);
   if (G.traversal == kUnitPerRay)
     launch_units_variant<kUnitPerRay>(G, v, blocks, lds, stream);
   else if (G.traversal == kUnitPacketScalar)
