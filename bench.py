@@ -58,7 +58,7 @@ def log(msg: str) -> None:
 
 def cpu_baseline(cfg: dict, spp: int, pose12: np.ndarray, target_s: float = 10.0) -> dict:
     """Times the CPU oracle on a bounded sample of the same workload (eye 0)."""
-    import oracle  # the checker, timed here as the CPU baseline
+    import oracle  # the CPU oracle, timed here as the CPU baseline
 
     t0 = time.time()
     osc = oracle.OracleScene.from_scene(cfg["scene"])

@@ -3,8 +3,8 @@
 ctypes wrapper of ``liboracle.so`` (built from ``oracle.cpp`` by the Makefile
 here), the scalar C++ restatement of the reference's rendering path.  Only
 ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
-``bench.py`` may import this module, and only as the checker: nothing in the
-product library calls it.
+``bench.py`` may import this module, and only to check or time against it:
+nothing in the product library calls it.
 
 Functions and the reference code they restate are listed in oracle.cpp.
 """

@@ -2,9 +2,9 @@
 //
 // Scalar C++ restatement of the compound-eye rendering path of the reference
 // (BrainsOnBoard/compound-ray), used only by tests/, the smoke check and the
-// CPU-baseline leg of bench.py as the checker.  It re-derives every step from
-// the reference sources and the documented design rules, independently of the
-// GPU library:
+// CPU-baseline leg of bench.py, to check the GPU results against.  It
+// re-derives every step from the reference sources and the documented design
+// rules, independently of the GPU library:
 //   ray generation   libEyeRenderer3/shaders.cu:648-723 (generateOffsetRay,
 //                    rotatePoint, __raygen__ommatidium), FWHM ratio :53
 //   random streams   one stream per (ommatidium, sample) advanced per frame,
