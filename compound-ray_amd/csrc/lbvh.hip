@@ -366,63 +366,70 @@ inline int blocks_for(size_t n, int threads) { return (int)((n + threads - 1) / 
 // to a node referenced exactly once, and the leaves cover every triangle
 // exactly once.  A violation is reported as an error instead of letting a
 // kernel read out of bounds.
-bool validate_structure(const uint32_t* d_ids, int n, const WideNode* d_nodes, int node_count, hipStream_t stream,
-                        std::string* err) {
-  std::vector<uint32_t> ids((size_t)n);
-  std::vector<WideNode> nodes((size_t)node_count);
-  hipError_t e = hipMemcpyAsync(ids.data(), d_ids, sizeof(uint32_t) * (size_t)n, hipMemcpyDeviceToHost, stream);
-  if (e == hipSuccess && node_count > 0)
-    e = hipMemcpyAsync(nodes.data(), d_nodes, sizeof(WideNode) * (size_t)node_count, hipMemcpyDeviceToHost, stream);
-  if (e == hipSuccess) e = hipStreamSynchronize(stream);
-  if (e != hipSuccess) {
-    if (err) *err = std::string("lbvh: validation readback failed: ") + hipGetErrorString(e);
-    return false;
-  }
-  std::vector<uint8_t> seen((size_t)n, 0);
-  for (uint32_t v : ids) {
-    if (v >= (uint32_t)n || seen[v]) {
-      if (err) *err = "lbvh: sorted order is not a permutation";
-      return false;
-    }
-    seen[v] = 1;
-  }
-  std::vector<uint32_t> refs((size_t)node_count, 0);
-  std::vector<uint8_t> covered((size_t)n, 0);
-  for (int w = 0; w < node_count; ++w)
-    for (int s = 0; s < 4; ++s) {
-      const int32_t c = nodes[(size_t)w].child[s];
-      if (c == kChildEmpty) continue;
-      if (child_is_leaf(c)) {
-        const uint32_t f = leaf_first(c), k = leaf_count(c);
-        if ((uint64_t)f + k > (uint64_t)n) {
-          if (err) *err = "lbvh: leaf range out of bounds";
-          return false;
-        }
-        for (uint32_t q = f; q < f + k; ++q) {
-          if (covered[q]) {
-            if (err) *err = "lbvh: triangle referenced by two leaves";
-            return false;
-          }
-          covered[q] = 1;
-        }
-      } else {
-        if (c <= w || c >= node_count || refs[(size_t)c]++) {
-          if (err) *err = "lbvh: bad internal child index";
-          return false;
-        }
+// Structure check of a finished build, on the device: the sorted order is a
+// permutation, every leaf range lies inside the triangle array, every
+// triangle is in exactly one leaf, every internal child index points forward
+// and every node but the root has exactly one parent.  Traversal relies on
+// all of it to stay in bounds.  Violations set bits of `flags`.
+enum : uint32_t { kBadOrder = 1u, kBadLeaf = 2u, kTwoLeaves = 4u, kNoLeaf = 8u, kBadChild = 16u, kUnreached = 32u };
+
+__global__ void k_count_order(const uint32_t* __restrict__ ids, int n, uint32_t* __restrict__ seen,
+                              uint32_t* __restrict__ flags) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t v = ids[i];
+  if (v >= (uint32_t)n)
+    atomicOr(flags, kBadOrder);
+  else
+    atomicAdd(&seen[v], 1u);
+}
+
+__global__ void k_count_refs(const WideNode* __restrict__ nodes, int node_count, int n, uint32_t* __restrict__ refs,
+                             uint32_t* __restrict__ covered, uint32_t* __restrict__ flags) {
+  const int w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= node_count) return;
+  for (int s = 0; s < 4; ++s) {
+    const int32_t c = nodes[w].child[s];
+    if (c == kChildEmpty) continue;
+    if (child_is_leaf(c)) {
+      const uint32_t f = leaf_first(c), k = leaf_count(c);
+      if ((uint64_t)f + k > (uint64_t)n) {
+        atomicOr(flags, kBadLeaf);
+        continue;
       }
+      for (uint32_t q = f; q < f + k; ++q) atomicAdd(&covered[q], 1u);
+    } else if (c <= w || c >= node_count) {
+      atomicOr(flags, kBadChild);
+    } else {
+      atomicAdd(&refs[c], 1u);
     }
-  for (int w = 1; w < node_count; ++w)
-    if (refs[(size_t)w] != 1) {
-      if (err) *err = "lbvh: unreachable wide node";
-      return false;
-    }
-  for (int q = 0; q < n; ++q)
-    if (!covered[(size_t)q]) {
-      if (err) *err = "lbvh: triangle not covered by any leaf";
-      return false;
-    }
-  return true;
+  }
+}
+
+__global__ void k_check_counts(const uint32_t* __restrict__ seen, const uint32_t* __restrict__ covered, int n,
+                               const uint32_t* __restrict__ refs, int node_count, uint32_t* __restrict__ flags) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t bad = 0;
+  if (i < n) {
+    if (seen[i] != 1u) bad |= kBadOrder;
+    if (covered[i] == 0u) bad |= kNoLeaf;
+    if (covered[i] > 1u) bad |= kTwoLeaves;
+  }
+  if (i >= 1 && i < node_count) {
+    if (refs[i] == 0u) bad |= kUnreached;
+    if (refs[i] > 1u) bad |= kBadChild;
+  }
+  if (bad) atomicOr(flags, bad);
+}
+
+std::string structure_error(uint32_t bits) {
+  if (bits & kBadOrder) return "lbvh: sorted order is not a permutation";
+  if (bits & kBadLeaf) return "lbvh: leaf range out of bounds";
+  if (bits & kTwoLeaves) return "lbvh: triangle referenced by two leaves";
+  if (bits & kNoLeaf) return "lbvh: triangle not covered by any leaf";
+  if (bits & kBadChild) return "lbvh: bad internal child index";
+  if (bits & kUnreached) return "lbvh: unreachable wide node";
+  return "";
 }
 
 }  // namespace
@@ -465,6 +472,7 @@ bool build_device_scene(const float* d_pos, const float* d_col, const uint32_t* 
   float* bounds = nullptr;
   uint32_t* cpart = nullptr;
   int32_t* dist = nullptr;
+  uint32_t* vflags = nullptr;
   int32_t *srcs_a = nullptr, *srcs_b = nullptr, *depth = nullptr, *stack_bound = nullptr;
   uint32_t *icount = nullptr, *ioffs = nullptr;
   ScanScratch scan;
@@ -497,6 +505,7 @@ bool build_device_scene(const float* d_pos, const float* d_col, const uint32_t* 
       LB_STEP(hipMalloc(&bounds, sizeof(float) * 6 * (size_t)nb));
       LB_STEP(hipMalloc(&cpart, sizeof(uint32_t) * 6 * (size_t)blocks_for(n, 256)));
       LB_STEP(hipMalloc(&dist, sizeof(int32_t) * (size_t)max_nodes));
+      LB_STEP(hipMalloc(&vflags, sizeof(uint32_t)));
       LB_STEP(hipMalloc(&srcs_a, sizeof(int32_t) * (size_t)max_nodes));
       LB_STEP(hipMalloc(&srcs_b, sizeof(int32_t) * (size_t)max_nodes));
       LB_STEP(hipMalloc(&depth, sizeof(int32_t) * (size_t)max_nodes));
@@ -593,14 +602,31 @@ bool build_device_scene(const float* d_pos, const float* d_col, const uint32_t* 
       }
       if (!lvl_ok) break;
       node_total = base;
-      if (!validate_structure(vin, n, out->nodes, node_total, stream, err)) break;
+      // structure check (seen, covered: the sort keys' buffers, free after the
+      // hierarchy; refs: the per-level child counts, free after the collapse)
+      LB_STEP(hipMemsetAsync(codes_a, 0, sizeof(uint32_t) * (size_t)n, stream));
+      LB_STEP(hipMemsetAsync(codes_b, 0, sizeof(uint32_t) * (size_t)n, stream));
+      LB_STEP(hipMemsetAsync(icount, 0, sizeof(uint32_t) * (size_t)max_nodes, stream));
+      LB_STEP(hipMemsetAsync(vflags, 0, sizeof(uint32_t), stream));
+      hipLaunchKernelGGL(k_count_order, dim3(blocks_for(n, 256)), dim3(256), 0, stream, vin, n, codes_a, vflags);
+      hipLaunchKernelGGL(k_count_refs, dim3(blocks_for(node_total, 256)), dim3(256), 0, stream, out->nodes,
+                         node_total, n, icount, codes_b, vflags);
+      hipLaunchKernelGGL(k_check_counts, dim3(blocks_for(std::max(n, node_total), 256)), dim3(256), 0, stream,
+                         codes_a, codes_b, n, icount, node_total, vflags);
+      LB_STEP(hipGetLastError());
       hipLaunchKernelGGL(k_gather, dim3(blocks_for(n, 256)), dim3(256), 0, stream, d_pos, d_col, d_mode, vin, n,
                          out->tris, out->cols);
       LB_STEP(hipGetLastError());
       uint32_t cb[6];
+      uint32_t bad = 0;
       LB_STEP(hipMemcpyAsync(&host_sb, stack_bound, sizeof(int32_t), hipMemcpyDeviceToHost, stream));
       LB_STEP(hipMemcpyAsync(cb, cbounds, sizeof(cb), hipMemcpyDeviceToHost, stream));
+      LB_STEP(hipMemcpyAsync(&bad, vflags, sizeof(bad), hipMemcpyDeviceToHost, stream));
       LB_STEP(hipStreamSynchronize(stream));
+      if (bad) {
+        if (err) *err = structure_error(bad);
+        break;
+      }
       for (int a = 0; a < 3; ++a) {
         out->centroid_lo[a] = fkey_inv(cb[a]);
         out->centroid_hi[a] = fkey_inv(cb[3 + a]);
@@ -613,7 +639,7 @@ bool build_device_scene(const float* d_pos, const float* d_col, const uint32_t* 
   } while (0);
   for (void* p : {(void*)tb, (void*)cen, (void*)cbounds, (void*)codes_a, (void*)codes_b, (void*)ids_a,
                   (void*)ids_b, (void*)hist, (void*)left, (void*)right, (void*)parent, (void*)range,
-                  (void*)bounds, (void*)cpart, (void*)dist, (void*)srcs_a, (void*)srcs_b, (void*)depth, (void*)stack_bound,
+                  (void*)bounds, (void*)cpart, (void*)dist, (void*)vflags, (void*)srcs_a, (void*)srcs_b, (void*)depth, (void*)stack_bound,
                   (void*)icount, (void*)ioffs})
     if (p) (void)hipFree(p);
   scan_release(&scan);
