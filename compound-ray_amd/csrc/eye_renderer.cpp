@@ -1364,7 +1364,7 @@ void setSplitLaunches(bool enabled) { set_split_launches(enabled); }
 
 void setRenderTuning(int unitsPerGrab, int blocks) {
   State& s = st();
-  s.units_per_grab = std::max(0, std::min(unitsPerGrab, 1024));
+  s.units_per_grab = std::max(0, std::min(unitsPerGrab, kMaxUnitsPerGrab));
   s.unit_blocks = std::max(0, blocks);
 }
 

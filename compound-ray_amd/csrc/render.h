@@ -70,6 +70,8 @@ enum UnitTraversal : int { kUnitPacket = 0, kUnitPacketScalar = 1, kUnitPerRay =
 
 // Work counters a launch needs at U.queue: one per slice of the unit range.
 constexpr int kQueueWords = 8;
+// Largest grab of the unit renderer: lane g of a wave holds unit g's count.
+constexpr int kMaxUnitsPerGrab = 64;
 
 // Launches the unit renderer with U's launch shape.  The queue counter is
 // cleared by a kernel first.

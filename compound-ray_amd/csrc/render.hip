@@ -414,6 +414,18 @@ __device__ __forceinline__ void locate_unit(const UnitLaunch& U, uint64_t u, int
   *out = (uint64_t)*p * (uint64_t)U.omm_count + (uint64_t)*i;
 }
 
+// Bitonic sort of one value per lane, largest first, across the 64 lanes.
+__device__ __forceinline__ uint32_t wave_sort_desc(uint32_t v) {
+  const int lane = (int)(threadIdx.x & 63);
+  for (int k = 2; k <= 64; k <<= 1)
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      const uint32_t o = (uint32_t)__shfl_xor((int)v, j);
+      const bool take_max = ((lane & j) == 0) == ((lane & k) == 0);
+      v = take_max ? (o > v ? o : v) : (o < v ? o : v);
+    }
+  return v;
+}
+
 __device__ __forceinline__ int pow2_at_least(uint32_t n) {
   int s = 1;
   while ((uint32_t)s < n) s <<= 1;
@@ -466,6 +478,13 @@ __global__ __launch_bounds__(kTraceThreads) void k_render_units(SceneView S, Uni
       locate_unit(U, (uint64_t)base + (uint64_t)lane, &p, &i, &out);
       my_n = unit_count(U, out, i);
     }
+    // Rounds take the grab's units largest first: power-of-two segments in
+    // decreasing size tile the 64 lanes without gaps.  Sorted position q
+    // holds unit offset order[q] (units above 32 samples, which fill whole
+    // rounds, compare equal).  Output positions stay per unit.
+    const uint32_t key = wave_sort_desc(lane < G ? ((min(my_n, 64u) + 1u) << 6) | (uint32_t)lane : 0u);
+    const int order = (int)(key & 63u);
+    my_n = (uint32_t)__shfl((int)my_n, order);
     int g = 0;
     while (g < G) {
       // ---- build a round: which unit (slot) each lane serves ----
@@ -494,9 +513,10 @@ __global__ __launch_bounds__(kTraceThreads) void k_render_units(SceneView S, Uni
       }
       // ---- this lane's unit ----
       const int src = slot < 0 ? 0 : slot;
+      const int unit_off = __shfl(order, src);
       int p, i;
       uint64_t out;
-      locate_unit(U, (uint64_t)base + (uint64_t)src, &p, &i, &out);
+      locate_unit(U, (uint64_t)base + (uint64_t)unit_off, &p, &i, &out);
       const uint32_t n_src = (uint32_t)__shfl((int)my_n, src);  // every lane takes part in the shuffle
       const uint32_t n = slot < 0 ? 0u : n_src;
       const float inv = 1.0f / (float)(n > 0u ? n : 1u);
@@ -1061,6 +1081,7 @@ bool launch_render_units(const DeviceScene& scene, const UnitLaunch& U_in, hipSt
   UnitLaunch G = U;
   if (G.units_per_grab <= 0)
     G.units_per_grab = (int)std::min<uint64_t>(16, std::max<uint64_t>(1, units / (2 * resident_waves)));
+  G.units_per_grab = std::min(G.units_per_grab, kMaxUnitsPerGrab);
   const uint64_t need = (units + (uint64_t)G.units_per_grab - 1) / (uint64_t)G.units_per_grab;
   const uint64_t waves = (uint64_t)blocks * (kTraceThreads / 64);
   if (waves > need) blocks = (int)((need + (kTraceThreads / 64) - 1) / (kTraceThreads / 64));

@@ -74,6 +74,18 @@ def test_state_before_loading(hostlib):
     assert (b.x, b.y, b.z) == (0.0, 0.0, 0.0)
 
 
+def test_render_tuning_clamps_grab(hostlib):
+    """A wave holds one grab's sample counts in its 64 lanes: larger grabs are
+    clamped to 64, negative values mean automatic."""
+    try:
+        for asked, kept in ((200, 64), (64, 64), (7, 7), (-3, 0)):
+            hostlib.setRenderTuning(asked, 5)
+            assert er.render_tuning(hostlib)["units_per_grab"] == kept
+            assert er.render_tuning(hostlib)["blocks"] == 5
+    finally:
+        hostlib.setRenderTuning(0, 0)
+
+
 # ------------------------------------------------------------- glTF parse ----
 
 def write_doc(tmp_path, name, doc, blob=None, embed=False):

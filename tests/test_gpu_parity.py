@@ -137,8 +137,10 @@ def test_unrecorded_render_matches(lib, config1, oracle_c1):
     lib.setSampleRecording(True)
 
 
-@pytest.mark.parametrize("grab,blocks", [(1, 1), (3, 7), (16, 0)])
+@pytest.mark.parametrize("grab,blocks", [(1, 1), (3, 7), (16, 0), (200, 0)])
 def test_launch_shapes_same_result(lib, config1, oracle_c1, grab, blocks):
+    """Any launch shape gives the oracle's colours; grabs above 64 units are
+    clamped to 64 (a wave keeps one grab's sample counts in its lanes)."""
     omm = config1["eyes"][0][:400]
     setup_eye(lib, config1, omm=omm, spp=24)
     lib.setRenderTuning(grab, blocks)
