@@ -218,6 +218,13 @@ def bench_adaptive(lib, er, scenes, cfg, eyes, target: float = 0.01, n_test: int
         spp_u = int(math.ceil(spp_u * 1.1))
         e_u = uniform_err(spp_u)
 
+    # launch shape of both timed renders, autotuned on the uniform batch: the
+    # traversal and blocks apply to both; the adaptive passes pick their grab size
+    lib.gotoCamera(eyes[0])
+    lib.setCurrentEyeSamplesPerOmmatidium(spp_u)
+    er.autotune_np(lib, test[:256], repeats=1)
+    shape = er.render_tuning(lib)
+
     # adaptive error on the reference poses, budget calibrated to the target
     def adaptive_err(scale):
         got, spp_sum, units = [], 0, 0
@@ -296,6 +303,7 @@ def bench_adaptive(lib, er, scenes, cfg, eyes, target: float = 0.01, n_test: int
         "rms_abs_adaptive": e_a,
         "rms_reference": f"{n_ref} test poses x both eyes, uniform 4,096-spp render on the GPU",
         "budget_scale": scale,
+        "launch_shape": shape,
         "training": {"rows": int(len(L)), "label_spp": label_spp, "poses": n_train, "set_s": t_set,
                      "train_s": t_train, "epochs": len(losses), "loss_first": float(losses[0]),
                      "loss_last": float(losses[-1])},
@@ -367,6 +375,11 @@ def bench_large(lib, er, scenes, tmp: str, target: float = 0.01, n_ref: int = 10
         spp_u = int(math.ceil(spp_u * 1.1))
         e_u = err_of(uniform(spp_u)[1])
 
+    # launch shape of both timed renders, autotuned on the uniform batch
+    lib.setCurrentEyeSamplesPerOmmatidium(spp_u)
+    er.autotune_np(lib, identity, repeats=1)
+    shape = er.render_tuning(lib)
+
     def adaptive(scale, want_rgb=True):
         lib.setCurrentEyeFrameIndex(0)
         return er.render_adaptive_np(lib, identity, error_target=target, max_spp=512, budget_scale=scale,
@@ -407,6 +420,7 @@ def bench_large(lib, er, scenes, tmp: str, target: float = 0.01, n_ref: int = 10
         "rms_abs_adaptive": e_a,
         "rms_reference": f"first {n_ref} ommatidia of the batch, uniform 4,096-spp render on the GPU",
         "budget_scale": scale,
+        "launch_shape": shape,
         "bvh": bvh,
         "scene_load_s": load_s,
         "training": {"rows": int(len(L)), "label_spp": 1024, "seconds": train_s, "loss_first": float(losses[0]),

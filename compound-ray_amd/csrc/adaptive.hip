@@ -208,6 +208,9 @@ bool AdaptiveRenderer::render(const DeviceScene& scene, UnitLaunch U, const std:
   A.stats = d_stats_;
   A.samples = nullptr;
   A.queue = d_queue_;
+  // Grab size chosen per pass (0: automatic).  A tuned size comes from uniform
+  // batches; a grab of few pilot units would leave most of the wave's lanes idle.
+  A.units_per_grab = 0;
   if (!launch_render_units(scene, A, stream, err)) return false;
   const unsigned gb = (unsigned)((units + kThreads - 1) / kThreads);
   hipLaunchKernelGGL(k_features, dim3(gb), dim3(kThreads), 0, stream, U.omms, d_poses_, U.omm_count, (uint64_t)units,
@@ -298,6 +301,7 @@ bool AdaptiveRenderer::training_set(const DeviceScene& scene, UnitLaunch U, cons
   A.stats = d_stats_;
   A.samples = nullptr;
   A.queue = d_queue_;
+  A.units_per_grab = 0;  // automatic, as in render()
   if (!launch_render_units(scene, A, stream, err)) return false;
   const unsigned gb = (unsigned)((units + kThreads - 1) / kThreads);
   hipLaunchKernelGGL(k_features, dim3(gb), dim3(kThreads), 0, stream, U.omms, d_poses_, U.omm_count, (uint64_t)units,

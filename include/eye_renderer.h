@@ -230,7 +230,8 @@ void setSplitLaunches(bool enabled);
 /* Launch shape of the persistent unit renderer: ommatidia taken from the work
  * queue per grab (0 = automatic, at most 64: a wave holds one grab's sample
  * counts in its lanes), and blocks of 256 threads (0 = every resident slot
- * once). */
+ * once).  The adaptive passes keep the blocks and traversal but choose their
+ * own grab size. */
 void setRenderTuning(int unitsPerGrab, int blocks);
 /* Writes the current launch shape: units per grab, blocks, traversal mode. */
 int getRenderTuning(int* shape3);
