@@ -438,6 +438,8 @@ def main() -> int:
                     help="render loop only (no 4096-spp error check, no CPU baseline): for profiler runs")
     ap.add_argument("--skip-batched", action="store_true", help="skip the 4,096-pose batched configuration")
     ap.add_argument("--skip-adaptive", action="store_true", help="skip the learned adaptive sampling comparison")
+    ap.add_argument("--batched-only", action="store_true",
+                    help="run only the 4,096-pose batched configuration (profiler runs of the persistent renderer)")
     ap.add_argument("--large", action="store_true",
                     help="also run the 256-eye, 13.4M-triangle heterogeneous batch (several minutes)")
     ap.add_argument("--no-split", action="store_true", help="disable split launches (chunks of a unit on several waves)")
@@ -459,6 +461,13 @@ def main() -> int:
     lib.setTraversalMode(args.trace_mode)
     lib.setSplitLaunches(not args.no_split)
     tmp = tempfile.mkdtemp(prefix="eyebench-")
+    if args.batched_only:
+        out = {"batched": bench_batched(lib, er, scenes, tmp)}
+        if args.copy_probe > 0:
+            lib.runCopyProbe(int(args.copy_probe * 2 ** 20), 10)
+        lib.stop()
+        print(json.dumps(out))
+        return 0
     t0 = time.time()
     cfg = scenes.build_config(2, tmp)
     log(f"generated C2 scene ({cfg['scene'].tri_count()} triangles) in {time.time() - t0:.1f} s")

@@ -439,13 +439,15 @@ __device__ __forceinline__ int pow2_at_least(uint32_t n) {
 // wave stay busy when sample counts are small (adaptive pilots, uneven
 // counts).  Each unit's colour is the same canonical sum either way.
 //
-// Six waves per SIMD (at most 80 VGPRs): traversal is latency-bound, so
-// resident waves set the throughput.  Left free, the compiler takes 120-138
-// VGPRs (three or four waves).  What it moves to scratch instead is mostly unit
-// bookkeeping reloaded once per round; the scalar-node variant also reloads a
-// few values per traversal step and is still faster at six waves than at five.
+// Packet variants: six waves per SIMD (at most 80 VGPRs).  Traversal is
+// latency-bound, so resident waves set the throughput.  Left free, the
+// compiler takes 120-138 VGPRs (three or four waves); under the bound it moves
+// unit bookkeeping to scratch and reloads it outside the traversal loop.  The
+// per-ray variant's LDS stack columns allow few resident blocks anyway; four
+// waves (128 VGPRs) keep its traversal loop free of scratch.
 template <bool kStats, bool kSamples, int kTrav>
-__global__ __launch_bounds__(kTraceThreads, 6) void k_render_units(SceneView S, UnitLaunch U) {
+__global__ __launch_bounds__(kTraceThreads, kTrav == kUnitPerRay ? 4 : 6) void k_render_units(SceneView S,
+                                                                                             UnitLaunch U) {
   // LDS (per-ray traversal only): one stack column per lane, laid out
   // [depth][blockDim.x]; packets keep their stack in VGPR lanes
   extern __shared__ uint32_t unit_lds[];
