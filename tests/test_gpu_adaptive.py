@@ -47,6 +47,30 @@ def test_adaptive_pipeline_exact(lib, config1, oracle_c1):
     assert np.array_equal(rgb, want)
 
 
+@pytest.mark.parametrize("mode, grab, blocks", [(2, 1, 0), (0, 2, 64), (1, 64, 7)])
+def test_adaptive_independent_of_launch_shape(lib, config1, mode, grab, blocks):
+    """An installed launch shape (traversal mode, units per grab, blocks) leaves
+    the adaptive counts and colours unchanged: the passes keep the traversal
+    and blocks, choose their own grab size and sum in the canonical order."""
+    omm = config1["eyes"][0][:300]
+    load_c1(lib, config1, omm)
+    assert lib.mlpInit(11) == 0, er.last_error(lib)
+    poses6 = scenes.batch_poses(config1["scene"], 3, seed=21, extent=10.0)
+    lib.setCurrentEyeFrameIndex(5)
+    _, rgb0, counts0, total0 = er.render_adaptive_np(lib, poses6, error_target=0.02, max_spp=96)
+    lib.setTraversalMode(mode)
+    lib.setRenderTuning(grab, blocks)
+    try:
+        lib.setCurrentEyeFrameIndex(5)
+        _, rgb1, counts1, total1 = er.render_adaptive_np(lib, poses6, error_target=0.02, max_spp=96)
+    finally:
+        lib.setRenderTuning(0, 0)
+        lib.setTraversalMode(1)
+    assert total1 == total0
+    assert np.array_equal(counts1, counts0)
+    assert np.array_equal(rgb1, rgb0)
+
+
 def test_training_set_matches_oracle(lib, config1, oracle_c1):
     omm = config1["eyes"][0][:64]
     load_c1(lib, config1, omm)
