@@ -149,6 +149,23 @@ def rms_abs(a, b) -> float:
     return float(np.sqrt(np.mean((np.asarray(a, np.float64) - np.asarray(b, np.float64)) ** 2)))
 
 
+def calibrate_budget(err_at, target: float, iters: int = 6):
+    """Budget scale of the adaptive render whose RMS error is at most `target`
+    and at least 97 % of it (error ~ 1/sqrt(budget)), the way the uniform rate
+    is set: returns (scale, error, extra) of the smallest scale tried that met
+    the target, or of the last one tried when none did."""
+    scale, best, last = 1.0, None, None
+    for _ in range(iters):
+        e, extra = err_at(scale)
+        last = (scale, e, extra)
+        if e <= target and (best is None or scale < best[0]):
+            best = last
+        if 0.97 * target <= e <= target:
+            break
+        scale *= (e / target) ** 2 * (1.03 if e > target else 1.0)
+    return best if best is not None else last
+
+
 def bench_adaptive(lib, er, scenes, cfg, eyes, target: float = 0.01, n_test: int = 1024, n_train: int = 8192,
                    train_units: int = 2_000_000, label_spp: int = 1024, n_ref: int = 16) -> dict:
     """Learned adaptive sampling on the bee-scale scene and eyes: the sampler
@@ -238,13 +255,7 @@ def bench_adaptive(lib, er, scenes, cfg, eyes, target: float = 0.01, n_test: int
             units += rgb.shape[0] * rgb.shape[1]
         return rms_abs(np.stack(got), np.stack(ref)), spp_sum / units
 
-    scale = 1.0
-    e_a, spp_a_ref = adaptive_err(scale)
-    for _ in range(4):
-        if e_a <= target:
-            break
-        scale *= (e_a / target) ** 2 * 1.05
-        e_a, spp_a_ref = adaptive_err(scale)
+    scale, e_a, spp_a_ref = calibrate_budget(adaptive_err, target)
 
     # 4. timed: 1,024 poses x both eyes, adaptive vs uniform batched vs renderFrame
     def timed(fn, repeats=2):
@@ -385,15 +396,11 @@ def bench_large(lib, er, scenes, tmp: str, target: float = 0.01, n_ref: int = 10
         return er.render_adaptive_np(lib, identity, error_target=target, max_spp=512, budget_scale=scale,
                                      want_rgb=want_rgb)
 
-    scale = 1.0
-    _, rgb, _, total = adaptive(scale)
-    e_a = err_of(rgb)
-    for _ in range(4):
-        if e_a <= target:
-            break
-        scale *= (e_a / target) ** 2 * 1.05
+    def adaptive_err(scale):
         _, rgb, _, total = adaptive(scale)
-        e_a = err_of(rgb)
+        return err_of(rgb), total
+
+    scale, e_a, total = calibrate_budget(adaptive_err, target)
 
     def timed(fn, repeats=3):
         fn()

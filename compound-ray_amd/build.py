@@ -31,6 +31,7 @@ SOURCES = [
     "batch.cpp",
     "mlp.hip",
     "adaptive.hip",
+    "knn.cpp",
     "eye_renderer.cpp",
 ]
 

@@ -5,9 +5,13 @@
 // Per (pose, ommatidium) unit:
 //   1. pilot: `pilot` samples with statistics (mean, per-channel variance,
 //      hit fraction, inverse-depth mean and variance);
-//   2. features (12): world-space axis (3), acceptance angle / 0.05,
+//   2. features (16): world-space axis (3), acceptance angle / 0.05,
 //      pilot mean RGB (3), pilot RGB standard deviation (3), mean inverse
-//      depth 1/(1+t) (0 for a miss), its standard deviation;
+//      depth 1/(1+t) (0 for a miss), its standard deviation; and over the
+//      unit's kNeighbours nearest ommatidia (same pose): the mean and the
+//      maximum of their pilot standard deviation (RGB mean), the mean
+//      absolute difference of their pilot mean RGB from the unit's, and the
+//      mean of their inverse-depth standard deviation;
 //   3. the MLP predicts log2 of the samples the unit needs for an RMS error of
 //      `error_target` on its own: n_req = mean RGB variance / error^2;
 //   4. allocation minimising the total samples for a mean squared error of
@@ -23,6 +27,7 @@
 #include <vector>
 
 #include "device_scene.h"
+#include "knn.h"
 #include "mlp.h"
 #include "render.h"
 
@@ -35,6 +40,7 @@ struct AdaptiveConfig {
   int pilot = 8;
   int max_spp = 512;
   float budget_scale = 1.0f;   // multiplies every allocation (calibration knob)
+  const uint32_t* neighbours = nullptr;  // device, kNeighbours per ommatidium (nearest_ommatidia)
 };
 
 struct AdaptiveResult {
@@ -54,7 +60,7 @@ class AdaptiveRenderer {
   // `label_spp` further samples of the chosen units (unit_mask[pose][omm] != 0).
   bool training_set(const DeviceScene& scene, UnitLaunch U, const std::vector<Pose>& poses,
                     const std::vector<uint8_t>& unit_mask, int pilot, int label_spp, float error_target,
-                    std::vector<float>* features, std::vector<float>* labels, hipStream_t stream, std::string* err);
+                    const uint32_t* neighbours, std::vector<float>* features, std::vector<float>* labels, hipStream_t stream, std::string* err);
   // Pilot and top-up colours (r, g, b, samples) and top-up counts of the
   // first `units` units of the last render.
   bool copy_last(size_t units, float4* pilot, float4* topup, uint32_t* counts, std::string* err);
@@ -68,7 +74,7 @@ class AdaptiveRenderer {
   float4* d_col_b_ = nullptr;     // top-up colours
   float4* d_final_ = nullptr;     // combined colours
   float4* d_stats_ = nullptr;     // pilot statistics (2 per unit)
-  float* d_feat_ = nullptr;       // 12 per unit
+  float* d_feat_ = nullptr;       // kMlpIn per unit
   float* d_pred_ = nullptr;       // per unit
   uint32_t* d_counts_ = nullptr;  // final sample count per unit
   uint32_t* d_extra_ = nullptr;   // top-up samples per unit

@@ -13,7 +13,7 @@ constexpr int kThreads = 256;
 
 __global__ void k_features(const Ommatidium* __restrict__ omms, const Pose* __restrict__ poses, int omm_count,
                            uint64_t units, const float4* __restrict__ col, const float4* __restrict__ stats,
-                           float* __restrict__ feat) {
+                           const uint32_t* __restrict__ nbr, float* __restrict__ feat) {
   const uint64_t u = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (u >= units) return;
   const int p = (int)(u / (uint64_t)omm_count);
@@ -39,6 +39,25 @@ __global__ void k_features(const Ommatidium* __restrict__ omms, const Pose* __re
   f[9] = sqrtf(s0.z);
   f[10] = s1.x;
   f[11] = sqrtf(s1.y);
+  // pooled pilot statistics of the nearest ommatidia at the same pose
+  const uint64_t row = (uint64_t)p * (uint64_t)omm_count;
+  float s_sum = 0.0f, s_max = 0.0f, contrast = 0.0f, d_sum = 0.0f;
+  for (int m = 0; m < kNeighbours; ++m) {
+    const uint64_t v = row + nbr[(size_t)i * kNeighbours + m];
+    const float4 cv = col[v];
+    const float4 a = stats[2 * v];
+    const float4 b = stats[2 * v + 1];
+    const float sv = (sqrtf(a.x) + sqrtf(a.y) + sqrtf(a.z)) * (1.0f / 3.0f);
+    s_sum += sv;
+    s_max = fmaxf(s_max, sv);
+    contrast += (fabsf(cv.x - c.x) + fabsf(cv.y - c.y) + fabsf(cv.z - c.z)) * (1.0f / 3.0f);
+    d_sum += sqrtf(b.y);
+  }
+  constexpr float kInvK = 1.0f / (float)kNeighbours;
+  f[12] = s_sum * kInvK;
+  f[13] = s_max;
+  f[14] = contrast * kInvK;
+  f[15] = d_sum * kInvK;
 }
 
 __device__ inline uint32_t lane_footprint(uint32_t e) {
@@ -190,6 +209,10 @@ bool AdaptiveRenderer::render(const DeviceScene& scene, UnitLaunch U, const std:
     if (err) *err = "adaptive: need 1 <= pilot <= max samples and positive error targets";
     return false;
   }
+  if (!cfg.neighbours) {
+    if (err) *err = "adaptive: no neighbour table";
+    return false;
+  }
   if (!ensure(np, units, err)) return false;
   if (hipMemcpyAsync(d_poses_, poses.data(), sizeof(Pose) * np, hipMemcpyHostToDevice, stream) != hipSuccess) {
     if (err) *err = "adaptive: pose upload failed";
@@ -214,7 +237,7 @@ bool AdaptiveRenderer::render(const DeviceScene& scene, UnitLaunch U, const std:
   if (!launch_render_units(scene, A, stream, err)) return false;
   const unsigned gb = (unsigned)((units + kThreads - 1) / kThreads);
   hipLaunchKernelGGL(k_features, dim3(gb), dim3(kThreads), 0, stream, U.omms, d_poses_, U.omm_count, (uint64_t)units,
-                     d_col_a_, d_stats_, d_feat_);
+                     d_col_a_, d_stats_, cfg.neighbours, d_feat_);
   if (!ok_launch("features", err)) return false;
   if (!mlp.predict(d_feat_, (int64_t)units, d_pred_, stream, err)) return false;
   hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, stream, d_total_);
@@ -266,12 +289,12 @@ bool AdaptiveRenderer::render(const DeviceScene& scene, UnitLaunch U, const std:
 
 bool AdaptiveRenderer::training_set(const DeviceScene& scene, UnitLaunch U, const std::vector<Pose>& poses,
                                     const std::vector<uint8_t>& unit_mask, int pilot, int label_spp,
-                                    float error_target, std::vector<float>* features, std::vector<float>* labels,
+                                    float error_target, const uint32_t* neighbours, std::vector<float>* features, std::vector<float>* labels,
                                     hipStream_t stream, std::string* err) {
   const size_t np = poses.size();
   const size_t units = np * (size_t)U.omm_count;
   if (units == 0) return true;
-  if (unit_mask.size() != units || pilot < 1 || label_spp < 2 || !(error_target > 0.0f)) {
+  if (unit_mask.size() != units || pilot < 1 || label_spp < 2 || !(error_target > 0.0f) || !neighbours) {
     if (err) *err = "adaptive: bad training-set request";
     return false;
   }
@@ -280,13 +303,11 @@ bool AdaptiveRenderer::training_set(const DeviceScene& scene, UnitLaunch U, cons
     if (err) *err = "adaptive: pose upload failed";
     return false;
   }
-  std::vector<uint32_t> pc(units), lc(units);
-  for (size_t u = 0; u < units; ++u) {
-    pc[u] = unit_mask[u] ? (uint32_t)pilot : 0u;
-    lc[u] = unit_mask[u] ? (uint32_t)label_spp : 0u;
-  }
-  if (hipMemcpyAsync(d_counts_, pc.data(), sizeof(uint32_t) * units, hipMemcpyHostToDevice, stream) != hipSuccess ||
-      hipMemcpyAsync(d_extra_, lc.data(), sizeof(uint32_t) * units, hipMemcpyHostToDevice, stream) != hipSuccess) {
+  // The pilot covers every unit (the neighbour features read the pilots of
+  // unmasked units too); only the label render is restricted to the mask.
+  std::vector<uint32_t> lc(units);
+  for (size_t u = 0; u < units; ++u) lc[u] = unit_mask[u] ? (uint32_t)label_spp : 0u;
+  if (hipMemcpyAsync(d_extra_, lc.data(), sizeof(uint32_t) * units, hipMemcpyHostToDevice, stream) != hipSuccess) {
     if (err) *err = "adaptive: count upload failed";
     return false;
   }
@@ -294,8 +315,9 @@ bool AdaptiveRenderer::training_set(const DeviceScene& scene, UnitLaunch U, cons
   A.poses = d_poses_;
   A.pose_count = (int)np;
   A.pose_base = 0;
+  A.spp = pilot;
   A.omm_counts = nullptr;
-  A.unit_counts = d_counts_;
+  A.unit_counts = nullptr;
   A.sample_base = 0;
   A.colours = d_col_a_;
   A.stats = d_stats_;
@@ -305,7 +327,7 @@ bool AdaptiveRenderer::training_set(const DeviceScene& scene, UnitLaunch U, cons
   if (!launch_render_units(scene, A, stream, err)) return false;
   const unsigned gb = (unsigned)((units + kThreads - 1) / kThreads);
   hipLaunchKernelGGL(k_features, dim3(gb), dim3(kThreads), 0, stream, U.omms, d_poses_, U.omm_count, (uint64_t)units,
-                     d_col_a_, d_stats_, d_feat_);
+                     d_col_a_, d_stats_, neighbours, d_feat_);
   if (!ok_launch("features", err)) return false;
   // label render: independent samples pilot .. pilot + label_spp - 1
   UnitLaunch B = A;

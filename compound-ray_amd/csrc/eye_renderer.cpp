@@ -19,6 +19,7 @@
 #include "adaptive.h"
 #include "batch.h"
 #include "device_scene.h"
+#include "knn.h"
 #include "mlp.h"
 #include "render.h"
 #include "projection.h"
@@ -67,6 +68,10 @@ struct EyeState {
   int proj_w = -1, proj_h = -1, proj_variant = -1;
   uint64_t proj_version = ~0ull;
   uint64_t omm_version = 0;   // bumped whenever the ommatidia change
+  // nearest-ommatidium table of the learned sampler, for the ommatidia of nbr_version
+  uint32_t* d_nbr = nullptr;
+  size_t d_nbr_cap = 0;
+  uint64_t nbr_version = ~0ull;
 
   uint64_t total_samples() const {
     if (counts.empty()) return (uint64_t)omms.size() * (uint64_t)spp;
@@ -210,6 +215,7 @@ void free_scene() {
     if (e.d_counts) (void)hipFree(e.d_counts);
     if (e.d_sample_offsets) (void)hipFree(e.d_sample_offsets);
     if (e.d_order) (void)hipFree(e.d_order);
+    if (e.d_nbr) (void)hipFree(e.d_nbr);
   }
   s.eyes.clear();
   s.cams.clear();
@@ -487,6 +493,22 @@ struct DeviceTemp {
   }
 
 }  // namespace
+
+// The learned sampler's neighbour table on the device (kNeighbours per
+// ommatidium), rebuilt on the host when the eye's ommatidia have changed.
+static const uint32_t* neighbours_of(EyeState& e) {
+  State& s = st();
+  if (e.d_nbr && e.nbr_version == e.omm_version) return e.d_nbr;
+  const size_t n = e.omms.size();
+  std::vector<uint32_t> t(std::max<size_t>(n, 1) * kNeighbours, 0u);
+  nearest_ommatidia(reinterpret_cast<const float*>(e.omms.data()), n, kNeighbours, kNeighbourPosScale, t.data());
+  ensure_device(&e.d_nbr, &e.d_nbr_cap, t.size());
+  hip_check(hipMemcpyAsync(e.d_nbr, t.data(), sizeof(uint32_t) * t.size(), hipMemcpyHostToDevice, s.stream),
+            "upload neighbours");
+  hip_check(hipStreamSynchronize(s.stream), "sync");
+  e.nbr_version = e.omm_version;
+  return e.d_nbr;
+}
 
 extern "C" {
 
@@ -1258,6 +1280,15 @@ int mlpTrain(const float* features, const float* targets, size_t rows, int epoch
 }
 
 
+int nearestOmmatidia(const float* ommatidia, size_t count, int k, uint32_t* out) {
+  ABI_GUARD_BEGIN
+  if (k < 1 || k > 16) throw std::runtime_error("nearestOmmatidia: k must be 1..16");
+  if (count && (!ommatidia || !out)) throw std::runtime_error("nearestOmmatidia: null array");
+  nearest_ommatidia(ommatidia, count, k, kNeighbourPosScale, out);
+  return 0;
+  ABI_GUARD_END(return -1)
+}
+
 int64_t adaptiveTrainingSet(const float* poses, size_t poseCount, const uint8_t* unitMask, int pilot, int labelSpp,
                             float errorTarget, float* features, float* labels, size_t maxRows) {
   ABI_GUARD_BEGIN
@@ -1272,8 +1303,8 @@ int64_t adaptiveTrainingSet(const float* poses, size_t poseCount, const uint8_t*
   std::vector<float> f, l;
   std::string err;
   UnitLaunch U = unit_launch_of(*e, nullptr, (int)poseCount);
-  if (!s.adaptive.training_set(s.scene, U, poses_from6(poses, poseCount), mask, pilot, labelSpp, errorTarget, &f,
-                               &l, s.stream, &err))
+  if (!s.adaptive.training_set(s.scene, U, poses_from6(poses, poseCount), mask, pilot, labelSpp, errorTarget,
+                               neighbours_of(*e), &f, &l, s.stream, &err))
     throw std::runtime_error(err);
   s.label_error = errorTarget;
   const size_t rows = std::min(l.size(), maxRows);
@@ -1296,6 +1327,7 @@ double renderPoseBatchAdaptive(const float* poses, size_t poseCount, float error
   cfg.label_error = s.label_error;
   cfg.max_spp = maxSpp;
   cfg.budget_scale = budgetScale;
+  cfg.neighbours = neighbours_of(*e);
   AdaptiveResult res;
   std::string err;
   UnitLaunch U = unit_launch_of(*e, nullptr, (int)poseCount);

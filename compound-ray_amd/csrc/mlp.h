@@ -1,9 +1,9 @@
-// Learned adaptive sampler: a 12 -> 64 -> 64 -> 64 -> 1 ReLU MLP in BF16 on
+// Learned adaptive sampler: a 16 -> 64 -> 64 -> 64 -> 1 ReLU MLP in BF16 on
 // MFMA with FP32 accumulation, with hand-written forward, backward and Adam
 // kernels for gfx950.  No reference counterpart (the reference samples every
 // ommatidium uniformly, libEyeRenderer.cpp:167-176).
 //
-// Parameters (FP32 master copy, this order): W1[12][64], b1[64], W2[64][64],
+// Parameters (FP32 master copy, this order): W1[16][64], b1[64], W2[64][64],
 // b2[64], W3[64][64], b3[64], W4[64], b4 — W[in][out].
 // Rounding points (the oracle mirrors them): inputs, weights, hidden
 // activations and back-propagated pre-activation gradients are rounded to
@@ -19,13 +19,13 @@
 
 namespace eye {
 
-constexpr int kMlpIn = 12;
+constexpr int kMlpIn = 16;
 constexpr int kMlpHidden = 64;
-constexpr int kMlpParams = 12 * 64 + 64 + 64 * 64 + 64 + 64 * 64 + 64 + 64 + 1;  // 9217
+constexpr int kMlpParams = kMlpIn * 64 + 64 + 64 * 64 + 64 + 64 * 64 + 64 + 64 + 1;  // 9473
 
 struct MlpOffsets {
   static constexpr int W1 = 0;
-  static constexpr int b1 = W1 + 12 * 64;
+  static constexpr int b1 = W1 + kMlpIn * 64;
   static constexpr int W2 = b1 + 64;
   static constexpr int b2 = W2 + 64 * 64;
   static constexpr int W3 = b2 + 64;
@@ -40,7 +40,7 @@ class Mlp {
   bool init(uint32_t seed, hipStream_t stream, std::string* err);
   bool set_params(const float* host, hipStream_t stream, std::string* err);
   bool get_params(float* host, hipStream_t stream, std::string* err);
-  // y[n] = f(X[n][12]); device pointers.
+  // y[n] = f(X[n][kMlpIn]); device pointers.
   bool predict(const float* d_x, int64_t n, float* d_y, hipStream_t stream, std::string* err);
   // Mean squared error and its gradient (FP32, kMlpParams) over rows
   // [0, n) of device arrays; *loss and grad are device outputs.

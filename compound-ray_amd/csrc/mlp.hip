@@ -22,7 +22,7 @@ constexpr int kWaves = kThreads / 64;
 constexpr int kGradBlocks = 64;
 
 // packed BF16 layouts (uint16 units)
-constexpr int P_W1T = 0;                 // [64 out][16 in]  (in padded 12 -> 16)
+constexpr int P_W1T = 0;                 // [64 out][16 in]
 constexpr int P_W2T = P_W1T + 64 * 16;   // [64 out][64 in]
 constexpr int P_W3T = P_W2T + 64 * 64;
 constexpr int P_W2N = P_W3T + 64 * 64;   // [64 in][64 out]

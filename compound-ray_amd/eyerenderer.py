@@ -18,6 +18,9 @@ from ctypes import (POINTER, Structure, c_bool, c_char_p, c_double, c_float, c_i
 import numpy as np
 from numpy.ctypeslib import ndpointer
 
+MLP_IN = 16      # features per unit of the learned sampler
+NEIGHBOURS = 6   # nearest ommatidia pooled into those features
+
 
 class c_float3(Structure):
     """The 12-byte float triple returned by getGeometry{Max,Min}Bounds."""
@@ -129,6 +132,8 @@ def configure_extensions(lib):
     lib.adaptiveTrainingSet.argtypes = [c_void_p, c_size_t, c_void_p, c_int, c_int, c_float, c_void_p, c_void_p,
                                         c_size_t]
     lib.adaptiveTrainingSet.restype = c_int64
+    lib.nearestOmmatidia.argtypes = [c_void_p, c_size_t, c_int, c_void_p]
+    lib.nearestOmmatidia.restype = c_int
     lib.renderPoseBatchAdaptive.argtypes = [c_void_p, c_size_t, c_float, c_int, c_float, c_void_p, c_void_p,
                                             c_void_p]
     lib.renderPoseBatchAdaptive.restype = c_double
@@ -341,13 +346,23 @@ def autotune_np(lib, poses6: np.ndarray, repeats: int = 1) -> np.ndarray:
     return rows[:n]
 
 
+def nearest_ommatidia_np(lib, omm: np.ndarray, k: int = NEIGHBOURS) -> np.ndarray:
+    """(n, k) indices of each ommatidium's k nearest others (the learned
+    sampler's neighbour table; host computation, no GPU)."""
+    o = np.ascontiguousarray(omm, dtype=np.float32).reshape(-1, 8)
+    out = np.zeros((o.shape[0], k), np.uint32)
+    if lib.nearestOmmatidia(o.ctypes.data_as(c_void_p), o.shape[0], k, out.ctypes.data_as(c_void_p)) != 0:
+        raise RuntimeError(last_error(lib))
+    return out
+
+
 def training_set_np(lib, poses6, unit_mask=None, pilot=8, label_spp=1024, error_target=0.01):
-    """(features (n, 12), labels (n,)) for the masked (pose, ommatidium) units."""
+    """(features (n, MLP_IN), labels (n,)) for the masked (pose, ommatidium) units."""
     p = np.ascontiguousarray(poses6, dtype=np.float32).reshape(-1, 6)
     n_omm = lib.getCurrentEyeOmmatidialCount()
     m = None if unit_mask is None else np.ascontiguousarray(unit_mask, dtype=np.uint8).reshape(-1)
     rows = int(m.sum()) if m is not None else p.shape[0] * n_omm
-    f = np.zeros((rows, 12), np.float32)
+    f = np.zeros((rows, MLP_IN), np.float32)
     lab = np.zeros(rows, np.float32)
     got = lib.adaptiveTrainingSet(p.ctypes.data_as(c_void_p), p.shape[0],
                                   None if m is None else m.ctypes.data_as(c_void_p), pilot, label_spp,

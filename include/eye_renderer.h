@@ -160,8 +160,8 @@ int traceRays(const float* rays, size_t count, int32_t* primIds, float* tbb);
  * Pose k uses counter word 3 = k.  Returns GPU milliseconds, or -1 on error. */
 double renderPoseBatch(const float* poses, size_t poseCount, float* rgbOut);
 
-/* ---- Learned sampler MLP (12 -> 64 -> 64 -> 64 -> 1, BF16 on MFMA) ----
- * Parameters are FP32 in the order W1[12][64], b1[64], W2[64][64], b2[64],
+/* ---- Learned sampler MLP (16 -> 64 -> 64 -> 64 -> 1, BF16 on MFMA) ----
+ * Parameters are FP32 in the order W1[16][64], b1[64], W2[64][64], b2[64],
  * W3[64][64], b3[64], W4[64], b4.  Host arrays; all return 0 or -1. */
 size_t mlpParamCount(void);
 int mlpInit(uint32_t seed);
@@ -176,12 +176,21 @@ int mlpTrain(const float* features, const float* targets, size_t rows, int epoch
 
 /* ---- Learned adaptive sampling of the current eye ----
  * Training rows for the sampler: for every (pose, ommatidium) with
- * unitMask[pose * ommatidia + omm] != 0 (NULL mask: all), 12 features from
- * `pilot` samples and the label log2(max(1, mean RGB variance / errorTarget^2))
- * from `labelSpp` further samples.  Writes at most maxRows rows (features
- * 12 floats each); returns the number of rows, or -1. */
+ * unitMask[pose * ommatidia + omm] != 0 (NULL mask: all), 16 features from
+ * `pilot` samples (the unit's own statistics and those of its 6 nearest
+ * ommatidia, nearestOmmatidia; the pilot covers every unit of the poses) and
+ * the label log2(max(1, mean RGB variance / errorTarget^2)) from `labelSpp`
+ * further samples.  Writes at most maxRows rows (features 16 floats each);
+ * returns the number of rows, or -1. */
 int64_t adaptiveTrainingSet(const float* poses, size_t poseCount, const uint8_t* unitMask, int pilot, int labelSpp,
                             float errorTarget, float* features, float* labels, size_t maxRows);
+/* Neighbour table of the learned sampler (no reference counterpart; host
+ * computation, no GPU): out[i * k + m] = the m-th nearest of the other `count`
+ * ommatidia (8 floats each, the OmmatidiumPacket layout) to ommatidium i under
+ * d^2 = sum over the unit axis and position / 0.01 of squared differences
+ * (FP64, in that coordinate order), ties to the lower index; slots beyond
+ * count - 1 neighbours repeat i.  k in 1..16; returns 0, or -1. */
+int nearestOmmatidia(const float* ommatidia, size_t count, int k, uint32_t* out);
 /* Batched adaptive render: 8-sample pilot, MLP prediction, per-pose Neyman
  * allocation for RMS error errorTarget (times budgetScale), top-up, combine.
  * The network predicts the samples needed for the label error (the

@@ -20,6 +20,7 @@ import numpy as np
 
 ORACLE_DIR = pathlib.Path(__file__).resolve().parent
 LIB_PATH = ORACLE_DIR / "liboracle.so"
+MLP_IN = 16  # inputs of the learned sampler's network
 
 _lib = None
 
@@ -210,7 +211,7 @@ def mlp_loss_grad(params, X, T=None, mode: int = 0):
     """FP64 MLP (mode 0 exact, mode 1 with the GPU's BF16 rounding points).
     Returns (y, grad or None, loss)."""
     p = np.ascontiguousarray(params, dtype=np.float32)
-    x = np.ascontiguousarray(X, dtype=np.float32).reshape(-1, 12)
+    x = np.ascontiguousarray(X, dtype=np.float32).reshape(-1, MLP_IN)
     n = x.shape[0]
     t = None if T is None else np.ascontiguousarray(T, dtype=np.float32)
     y = np.zeros(n, dtype=np.float64)

@@ -923,8 +923,8 @@ void oracle_pose(const float* pos, const float* rot, float* out) {
 }
 
 // ------------------------------------------------------ learned sampler ----
-// FP64 12-64-64-64-1 ReLU MLP with the GPU kernels' parameter order
-// (W1[12][64], b1, W2[64][64], b2, W3[64][64], b3, W4[64], b4; W[in][out]).
+// FP64 16-64-64-64-1 ReLU MLP with the GPU kernels' parameter order
+// (W1[16][64], b1, W2[64][64], b2, W3[64][64], b3, W4[64], b4; W[in][out]).
 // mode 0: exact FP64.  mode 1: BF16 rounding (nearest even) of the inputs,
 // weights, hidden activations and pre-activation gradients where the GPU
 // feeds them to MFMA, FP64 sums elsewhere.
@@ -937,7 +937,7 @@ static double bf16r(double x) {
 
 void oracle_mlp_loss_grad(const float* params, const float* X, const float* T, int64_t n, int mode, double* y_out,
                           double* grad, double* loss_out) {
-  const int H = 64, IN = 12;
+  const int H = 64, IN = 16;
   auto R = [&](double v) { return mode ? bf16r(v) : v; };
   const float* W1 = params;
   const float* b1 = W1 + IN * H;

@@ -76,7 +76,7 @@ def test_training_set_matches_oracle(lib, config1, oracle_c1):
     load_c1(lib, config1, omm)
     poses6 = scenes.batch_poses(config1["scene"], 2, seed=5, extent=10.0)
     feats, labels = er.training_set_np(lib, poses6, pilot=8, label_spp=256, error_target=0.01)
-    assert feats.shape == (128, 12) and labels.shape == (128,)
+    assert feats.shape == (128, er.MLP_IN) and labels.shape == (128,)
     poses12 = np.stack([oracle.pose12(p) for p in poses6])
     eye = lib.getCurrentEyeIndex()
     frame = lib.getCurrentEyeFrameIndex()
@@ -90,6 +90,17 @@ def test_training_set_matches_oracle(lib, config1, oracle_c1):
     assert np.allclose(feats[:, 7:10], std, atol=2e-3)
     invd = np.where(ps[..., 3] >= 0, 1.0 / (1.0 + ps[..., 3]), 0.0)
     assert np.allclose(feats[:, 10], invd.mean(axis=2).reshape(-1), atol=1e-5)
+    # pooled over the 6 nearest ommatidia (same pose): mean and max pilot
+    # standard deviation, colour contrast, spread of inverse depth
+    nbr = er.nearest_ommatidia_np(lib, omm).astype(np.int64)
+    f = feats.reshape(2, 64, er.MLP_IN)
+    s1 = np.sqrt(ps[..., :3].var(axis=2)).mean(axis=2)
+    col = pil["colour_gpu_order"].reshape(2, 64, 3).astype(np.float64)
+    sdi = np.sqrt(invd.var(axis=2))
+    assert np.allclose(f[..., 12], s1[:, nbr].mean(axis=2), atol=2e-3)
+    assert np.allclose(f[..., 13], s1[:, nbr].max(axis=2), atol=2e-3)
+    assert np.allclose(f[..., 14], np.abs(col[:, nbr, :] - col[:, :, None, :]).mean(axis=(2, 3)), atol=1e-5)
+    assert np.allclose(f[..., 15], sdi[:, nbr].mean(axis=2), atol=2e-3)
     # world-space axis
     for k, P in enumerate(poses12):
         a = omm[:, 3:6] / np.linalg.norm(omm[:, 3:6], axis=1, keepdims=True)
@@ -100,6 +111,23 @@ def test_training_set_matches_oracle(lib, config1, oracle_c1):
     var = ls[..., :3].var(axis=2).mean(axis=2).reshape(-1)
     want = np.clip(np.log2(np.maximum(var / 1e-4, 1.0)), 0, 12)
     assert np.allclose(labels, want, atol=2e-2)
+
+
+def test_training_set_mask_keeps_features(lib, config1):
+    """Masked rows carry the features and labels of the same units in an
+    unmasked request: the pilot covers every unit, so the pooled neighbour
+    statistics see unmasked neighbours too."""
+    omm = config1["eyes"][0][:200]
+    load_c1(lib, config1, omm)
+    poses6 = scenes.batch_poses(config1["scene"], 2, seed=5, extent=10.0)
+    lib.setCurrentEyeFrameIndex(9)
+    f_all, l_all = er.training_set_np(lib, poses6, pilot=8, label_spp=64, error_target=0.01)
+    mask = (np.random.default_rng(3).random((2, 200)) < 0.2).astype(np.uint8)
+    lib.setCurrentEyeFrameIndex(9)
+    f_m, l_m = er.training_set_np(lib, poses6, unit_mask=mask, pilot=8, label_spp=64, error_target=0.01)
+    sel = mask.reshape(-1) != 0
+    assert np.array_equal(f_m, f_all[sel])
+    assert np.array_equal(l_m, l_all[sel])
 
 
 def rms(a, b):

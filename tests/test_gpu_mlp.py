@@ -10,7 +10,8 @@ from compound_ray_amd import eyerenderer as er
 
 pytestmark = pytest.mark.gpu
 
-P = 9217
+P = 16 * 64 + 64 + 64 * 64 + 64 + 64 * 64 + 64 + 64 + 1  # 9473: W1[16][64], b1, W2, b2, W3, b3, W4, b4
+IN = er.MLP_IN
 
 
 def params_of(lib):
@@ -52,7 +53,7 @@ def mlp(lib):
 @pytest.mark.parametrize("n", [1, 15, 16, 17, 1000, 40000])
 def test_forward_matches_oracle(mlp, n):
     rng = np.random.default_rng(n)
-    x = rng.normal(size=(n, 12)).astype(np.float32)
+    x = rng.normal(size=(n, IN)).astype(np.float32)
     p = params_of(mlp)
     y = predict(mlp, x)
     want, _, _ = oracle.mlp_loss_grad(p, x, mode=1)
@@ -63,15 +64,16 @@ def test_forward_matches_oracle(mlp, n):
 @pytest.mark.parametrize("n", [16, 50, 513, 8192])
 def test_gradient_matches_oracle(mlp, n):
     rng = np.random.default_rng(100 + n)
-    x = rng.normal(size=(n, 12)).astype(np.float32)
+    x = rng.normal(size=(n, IN)).astype(np.float32)
     t = rng.normal(size=n).astype(np.float32)
     p = params_of(mlp)
     loss, g = loss_grad(mlp, x, t)
     _, want, wloss = oracle.mlp_loss_grad(p, x, t, mode=1)
     assert abs(loss - wloss) <= 1e-3 * max(1.0, wloss)
     # per parameter block: relative error of the gradient vector
-    blocks = [(0, 768), (768, 832), (832, 4928), (4928, 4992), (4992, 9088), (9088, 9152), (9152, 9216),
-              (9216, 9217)]
+    w1 = IN * 64
+    blocks = [(0, w1), (w1, w1 + 64), (w1 + 64, w1 + 4160), (w1 + 4160, w1 + 4224), (w1 + 4224, w1 + 8320),
+              (w1 + 8320, w1 + 8384), (w1 + 8384, w1 + 8448), (w1 + 8448, w1 + 8449)]
     for a, b in blocks:
         num = np.linalg.norm(g[a:b] - want[a:b])
         den = np.linalg.norm(want[a:b]) + 1e-12
@@ -81,7 +83,7 @@ def test_gradient_matches_oracle(mlp, n):
 def test_training_fits_a_smooth_function(mlp):
     rng = np.random.default_rng(9)
     n = 20000
-    x = rng.uniform(-1, 1, size=(n, 12)).astype(np.float32)
+    x = rng.uniform(-1, 1, size=(n, IN)).astype(np.float32)
     t = (np.sin(2 * x[:, 0]) + x[:, 1] * x[:, 2] + 0.5 * np.abs(x[:, 3])).astype(np.float32)
     assert mlp.mlpInit(3) == 0
     losses = np.zeros(30, np.float32)
@@ -91,7 +93,7 @@ def test_training_fits_a_smooth_function(mlp):
     assert losses[-1] < 0.05 * losses[0], losses
     # the trained network evaluates the same on the oracle
     p = params_of(mlp)
-    xt = rng.uniform(-1, 1, size=(2000, 12)).astype(np.float32)
+    xt = rng.uniform(-1, 1, size=(2000, IN)).astype(np.float32)
     want, _, _ = oracle.mlp_loss_grad(p, xt, mode=1)
     assert np.max(np.abs(predict(mlp, xt) - want)) < 2e-2
     tt = np.sin(2 * xt[:, 0]) + xt[:, 1] * xt[:, 2] + 0.5 * np.abs(xt[:, 3])

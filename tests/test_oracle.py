@@ -387,7 +387,7 @@ def test_monte_carlo_convergence(config1, oracle_c1):
 
 def mlp_params(seed=0, scale=1.0):
     rng = np.random.default_rng(seed)
-    sizes = [(12 * 64, 12), (64, 0), (64 * 64, 64), (64, 0), (64 * 64, 64), (64, 0), (64, 64), (1, 0)]
+    sizes = [(oracle.MLP_IN * 64, oracle.MLP_IN), (64, 0), (64 * 64, 64), (64, 0), (64 * 64, 64), (64, 0), (64, 64), (1, 0)]
     parts = []
     for n, fan in sizes:
         lim = math.sqrt(6.0 / fan) if fan else 0.1
@@ -400,7 +400,7 @@ def test_mlp_backprop_matches_finite_differences():
     (which pins the backward pass the GPU gradients are compared with)."""
     rng = np.random.default_rng(1)
     p = mlp_params(1)
-    x = rng.normal(size=(40, 12)).astype(np.float32)
+    x = rng.normal(size=(40, oracle.MLP_IN)).astype(np.float32)
     t = rng.normal(size=40).astype(np.float32)
     _, g, loss = oracle.mlp_loss_grad(p, x, t, mode=0)
     assert loss > 0
@@ -425,7 +425,7 @@ def test_mlp_backprop_matches_finite_differences():
 def test_mlp_bf16_mode_close_to_exact():
     rng = np.random.default_rng(2)
     p = mlp_params(2)
-    x = rng.normal(size=(200, 12)).astype(np.float32)
+    x = rng.normal(size=(200, oracle.MLP_IN)).astype(np.float32)
     t = rng.normal(size=200).astype(np.float32)
     y0, g0, _ = oracle.mlp_loss_grad(p, x, t, mode=0)
     y1, g1, _ = oracle.mlp_loss_grad(p, x, t, mode=1)
