@@ -445,6 +445,8 @@ def main() -> int:
                     help="render loop only (no 4096-spp error check, no CPU baseline): for profiler runs")
     ap.add_argument("--skip-batched", action="store_true", help="skip the 4,096-pose batched configuration")
     ap.add_argument("--skip-adaptive", action="store_true", help="skip the learned adaptive sampling comparison")
+    ap.add_argument("--adaptive-only", action="store_true",
+                    help="run only the learned adaptive comparison on the bee-scale scene (profiler runs)")
     ap.add_argument("--batched-only", action="store_true",
                     help="run only the 4,096-pose batched configuration (profiler runs of the persistent renderer)")
     ap.add_argument("--large", action="store_true",
@@ -494,6 +496,11 @@ def main() -> int:
             lib.setCameraPose(*[float(v) for v in cfg["pose"]])
             eyes.append(i)
     assert len(eyes) == 2
+    if args.adaptive_only:
+        out = {"learned_adaptive": bench_adaptive(lib, er, scenes, cfg, eyes)}
+        lib.stop()
+        print(json.dumps(out))
+        return 0
     rays_per_step = 0
     stats = {"nodes_visited": 0, "tris_tested": 0, "hits": 0}
     omm_counts = []
