@@ -251,6 +251,9 @@ bool AdaptiveRenderer::render(const DeviceScene& scene, UnitLaunch U, const std:
   B.sample_base = (uint32_t)cfg.pilot;
   B.colours = d_col_b_;
   B.stats = nullptr;
+  // Top-up grabs of at most 8 units: a wave's rounds then mix the cones of
+  // fewer, closer ommatidia (measured 1-2 % faster than 16 on the bee eye).
+  B.auto_grab_max = 8;
   if (!launch_render_units(scene, B, stream, err)) return false;
   hipLaunchKernelGGL(k_combine, dim3(gb), dim3(kThreads), 0, stream, d_col_a_, d_col_b_, d_extra_, cfg.pilot,
                      (uint64_t)units, d_final_);

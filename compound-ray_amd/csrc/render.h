@@ -58,7 +58,8 @@ struct UnitLaunch {
   uint64_t chunk_done_cap = 0;          // counters
   // launch shape (results do not depend on it)
   int units_per_grab = 0;               // consecutive units a wave takes from the queue (0: chosen
-                                        //   from the launch size)
+                                        //   from the launch size, at most auto_grab_max)
+  int auto_grab_max = 16;
   int blocks = 0;                       // blocks of 256 threads; 0: every resident slot once
   int traversal = 0;                    // UnitTraversal
 };

@@ -1085,10 +1085,11 @@ bool launch_render_units(const DeviceScene& scene, const UnitLaunch& U_in, hipSt
     return check_launch("render_chunks", err);
   }
   // units per queue grab: as given, or (0) as many as keep two grabs per
-  // resident wave, at most 16
+  // resident wave, at most auto_grab_max
   UnitLaunch G = U;
   if (G.units_per_grab <= 0)
-    G.units_per_grab = (int)std::min<uint64_t>(16, std::max<uint64_t>(1, units / (2 * resident_waves)));
+    G.units_per_grab = (int)std::min<uint64_t>((uint64_t)std::max(G.auto_grab_max, 1),
+                                               std::max<uint64_t>(1, units / (2 * resident_waves)));
   G.units_per_grab = std::min(G.units_per_grab, kMaxUnitsPerGrab);
   const uint64_t need = (units + (uint64_t)G.units_per_grab - 1) / (uint64_t)G.units_per_grab;
   const uint64_t waves = (uint64_t)blocks * (kTraceThreads / 64);
