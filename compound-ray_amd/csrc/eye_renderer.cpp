@@ -8,6 +8,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <iostream>
@@ -245,15 +246,39 @@ void upload_eye(EyeState& e) {
   } else {
     for (size_t i = 0; i < n; ++i) first[i] = (uint64_t)i * (uint64_t)e.spp;
   }
-  // Units are processed in the order of their axes' 3D Morton codes, so that
-  // waves running at the same time trace neighbouring view directions and
-  // share BVH nodes in cache.  Output positions are unchanged.
-  std::vector<std::pair<uint32_t, uint32_t>> key(n);
+  // Units are processed in the order of (position cell, axis Morton code):
+  // waves running at the same time trace neighbouring view directions from
+  // the same place and share BVH nodes in cache.  Cells are 1 scene unit (or
+  // 1/1024 of the positions' extent, if larger), so one eye (millimetres
+  // across) sorts by direction alone while eyes merged into one ommatidium
+  // list stay grouped.  Output positions are unchanged.
+  float plo[3] = {INFINITY, INFINITY, INFINITY}, phi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (size_t i = 0; i < n; ++i) {
+    const float p[3] = {e.omms[i].px, e.omms[i].py, e.omms[i].pz};
+    for (int a = 0; a < 3; ++a)
+      if (std::isfinite(p[a])) {
+        plo[a] = std::min(plo[a], p[a]);
+        phi[a] = std::max(phi[a], p[a]);
+      }
+  }
+  double ext = 0.0;
+  for (int a = 0; a < 3; ++a)
+    if (phi[a] >= plo[a]) ext = std::max(ext, (double)phi[a] - (double)plo[a]);
+  const double cell = std::max(1.0, ext / 1023.0);
+  auto cell_of = [&](float p, int a) -> uint32_t {
+    if (!std::isfinite(p) || !(phi[a] >= plo[a])) return 0u;
+    const double q = std::floor(((double)p - (double)plo[a]) / cell);
+    return (uint32_t)std::min(1023.0, std::max(0.0, q));
+  };
+  std::vector<std::pair<uint64_t, uint32_t>> key(n);
   for (size_t i = 0; i < n; ++i) {
     const V3 d = normalize(v3(e.omms[i].dx, e.omms[i].dy, e.omms[i].dz));
     const float lo[3] = {-1.0f, -1.0f, -1.0f}, hi[3] = {1.0f, 1.0f, 1.0f};
     const bool ok = d.x == d.x && d.y == d.y && d.z == d.z;
-    key[i] = {ok ? morton30(d.x, d.y, d.z, lo, hi) : 0u, (uint32_t)i};
+    const uint64_t where = ((uint64_t)expand_bits10(cell_of(e.omms[i].px, 0)) << 2) |
+                           ((uint64_t)expand_bits10(cell_of(e.omms[i].py, 1)) << 1) |
+                           (uint64_t)expand_bits10(cell_of(e.omms[i].pz, 2));
+    key[i] = {(where << 30) | (ok ? morton30(d.x, d.y, d.z, lo, hi) : 0u), (uint32_t)i};
   }
   std::sort(key.begin(), key.end());
   std::vector<uint32_t> order(n);
