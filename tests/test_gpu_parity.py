@@ -236,6 +236,27 @@ def test_packed_small_units(lib, config1, oracle_c1, grab):
         er.set_sample_counts(lib, None)
 
 
+def test_merged_eyes_bit_exact(lib, config1, oracle_c1):
+    """Copies of an eye moved to world positions metres apart and merged into
+    one ommatidium list (units then sort by position cell before direction),
+    with counts from 1 to 79 (packed small units and whole-wave units): the
+    samples and colours are the oracle's."""
+    base = config1["eyes"][0][:120]
+    poses = scenes.batch_poses(config1["scene"], 4, seed=31, extent=10.0)
+    omm = np.concatenate([scenes.world_ommatidia(base, oracle.pose12(p)) for p in poses]).astype(np.float32)
+    assert np.ptp(omm[:, 0]) > 2.0 or np.ptp(omm[:, 2]) > 2.0
+    counts = np.random.default_rng(31).integers(1, 80, size=len(omm)).astype(np.uint32)
+    setup_eye(lib, config1, omm=omm, spp=4, pose6=np.zeros(6))
+    er.set_sample_counts(lib, counts)
+    try:
+        assert lib.renderFrame() >= 0, er.last_error(lib)
+        ref = oracle_c1.render(omm, er.pose_np(lib), counts=counts, want_samples=True)
+        assert np.array_equal(er.samples_np(lib).view(np.uint32), ref["samples"].view(np.uint32))
+        assert np.array_equal(er.colours_np(lib), ref["colour_gpu_order"][0])
+    finally:
+        er.set_sample_counts(lib, None)
+
+
 def test_pose_batch(lib, config1, oracle_c1):
     omm = config1["eyes"][0][:300]
     setup_eye(lib, config1, omm=omm, spp=16)
