@@ -21,6 +21,9 @@ constexpr double kNeighbourPosScale = 0.01;
 // position; FP64), in increasing (d^2, j) order.  Ommatidia with fewer than k
 // others repeat their own index in the remaining slots.  `omm` holds n records
 // of 8 floats (position, direction, acceptance angle, focal offset).
+// Cost: O(n log n) for eyes whose ommatidia are distinct; a cluster of m
+// exactly coincident ommatidia is one bucket that every query inside it
+// scans, O(m^2) in total.
 void nearest_ommatidia(const float* omm, size_t n, int k, double pos_scale, uint32_t* out);
 
 }  // namespace eye
