@@ -22,8 +22,8 @@ constexpr double kNeighbourPosScale = 0.01;
 // others repeat their own index in the remaining slots.  `omm` holds n records
 // of 8 floats (position, direction, acceptance angle, focal offset).
 // Cost: O(n log n) for eyes whose ommatidia are distinct; a cluster of m
-// exactly coincident ommatidia is one bucket that every query inside it
-// scans, O(m^2) in total.
+// exactly coincident ommatidia cannot be split by distance, and every query
+// inside it scans the whole cluster, O(m^2) in total.
 void nearest_ommatidia(const float* omm, size_t n, int k, double pos_scale, uint32_t* out);
 
 }  // namespace eye
