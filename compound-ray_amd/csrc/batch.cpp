@@ -131,7 +131,7 @@ void BatchRenderer::release() {
   ev0_ = ev1_ = nullptr;
   poses_cap_ = colours_cap_ = 0;
   g_valid_ = false;
-  g_scene_ = nullptr;
+  g_scene_ = 0;
 }
 
 bool BatchRenderer::build_graph(const DeviceScene& scene, const UnitLaunch& U, hipStream_t stream,
@@ -161,7 +161,7 @@ bool BatchRenderer::build_graph(const DeviceScene& scene, const UnitLaunch& U, h
   }
   g_launch_ = U;
   g_valid_ = true;
-  g_scene_ = scene.nodes;
+  g_scene_ = scene.generation;
   return true;
 }
 
@@ -246,7 +246,7 @@ bool BatchRenderer::render(const DeviceScene& scene, UnitLaunch U, const std::ve
   U.colours = d_colours_;
   U.queue = d_queue_;
   if (use_graph_) {
-    if (realloc || !exec_ || !g_valid_ || g_scene_ != scene.nodes || !same_launch(U, g_launch_))
+    if (realloc || !exec_ || !g_valid_ || g_scene_ != scene.generation || !same_launch(U, g_launch_))
       if (!build_graph(scene, U, stream, err)) return false;
     (void)hipEventRecord(ev0_, stream);
     if (hipGraphLaunch(exec_, stream) != hipSuccess) {

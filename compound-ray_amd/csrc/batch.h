@@ -83,7 +83,7 @@ class BatchRenderer {
   hipGraph_t graph_ = nullptr;
   hipGraphExec_t exec_ = nullptr;
   // shape the captured graph was built for
-  const void* g_scene_ = nullptr;
+  uint64_t g_scene_ = 0;  // generation of the scene the graph was captured on
   UnitLaunch g_launch_;
   bool g_valid_ = false;
 };

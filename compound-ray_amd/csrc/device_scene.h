@@ -35,6 +35,10 @@ struct DeviceScene {
   const TexView* textures = nullptr;
   int texture_count = 0;
   double build_ms = 0.0;
+  // Distinct for every scene the library builds (a process-wide counter set
+  // after each build): caches keyed on a scene compare it, because a rebuilt
+  // scene can reuse the freed buffers' addresses.
+  uint64_t generation = 0;
   float centroid_lo[3] = {0, 0, 0}, centroid_hi[3] = {0, 0, 0};
 };
 

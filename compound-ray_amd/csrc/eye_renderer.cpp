@@ -349,6 +349,8 @@ void build_scene_bvh() {
   if (!build_device_scene(s.d_pos, s.d_col, s.d_mode, (int)s.tri_count, s.leaf_max, s.stream, &s.scene,
                           &err))
     throw std::runtime_error(err);
+  static uint64_t generations = 0;
+  s.scene.generation = ++generations;
   s.scene.sky_mode = s.host.sky_mode;
   s.scene.texels = s.d_texels;
   s.scene.textures = s.d_textures;

@@ -32,7 +32,10 @@ struct JsonValue {
   size_t size() const { return kind == Array ? arr.size() : (kind == Object ? obj.size() : 0); }
 
   double as_number(double dflt = 0.0) const { return kind == Number ? num : dflt; }
-  int as_int(int dflt = -1) const { return kind == Number ? (int)num : dflt; }
+  // Truncated to int; values outside the int range (and NaN) give dflt.
+  int as_int(int dflt = -1) const {
+    return kind == Number && num > -2147483649.0 && num < 2147483648.0 ? (int)num : dflt;
+  }
   std::string as_string(const std::string& dflt = "") const { return kind == String ? str : dflt; }
 };
 

@@ -195,13 +195,31 @@ struct Loader {
     return true;
   }
 
+  // A byte count or offset from the JSON: absent gives 0; negative,
+  // fractional, non-numeric or huge (2^48 and above) values are rejected, so
+  // the bounds arithmetic below cannot wrap.
+  static bool json_size(const JsonValue& v, size_t* out) {
+    if (v.is_null()) {
+      *out = 0;
+      return true;
+    }
+    if (!v.is_number()) return false;
+    const double d = v.num;
+    if (!(d >= 0.0) || d != std::floor(d) || d >= 281474976710656.0) return false;
+    *out = (size_t)d;
+    return true;
+  }
+
   bool accessor(int idx, Accessor* a) {
     const JsonValue& acc = doc["accessors"][(size_t)idx];
     if (!acc.is_object()) {
       err = "accessor " + std::to_string(idx) + " missing";
       return false;
     }
-    a->count = (size_t)acc["count"].as_number(0.0);
+    if (!json_size(acc["count"], &a->count)) {
+      err = "accessor " + std::to_string(idx) + ": count must be a non-negative integer";
+      return false;
+    }
     a->ncomp = components_of(acc["type"].as_string());
     a->ctype = acc["componentType"].as_int(0);
     a->normalized = acc["normalized"].is_bool() && acc["normalized"].b;
@@ -221,16 +239,26 @@ struct Loader {
       err = "bufferView " + std::to_string(bv_idx) + ": bad buffer";
       return false;
     }
-    const size_t off = (size_t)bv["byteOffset"].as_number(0.0) + (size_t)acc["byteOffset"].as_number(0.0);
-    const size_t elem = (size_t)cb * (size_t)a->ncomp;
-    a->stride = (size_t)bv["byteStride"].as_number(0.0);
-    if (a->stride == 0) a->stride = elem;
-    const size_t need = a->count == 0 ? 0 : off + a->stride * (a->count - 1) + elem;
-    if (need > buffers[(size_t)buf].size()) {
-      err = "accessor " + std::to_string(idx) + ": reads past the end of its buffer";
+    size_t bv_off = 0, bv_len = 0, off = 0;
+    if (!json_size(bv["byteOffset"], &bv_off) || !json_size(acc["byteOffset"], &off) ||
+        !json_size(bv["byteStride"], &a->stride) || !json_size(bv["byteLength"], &bv_len)) {
+      err = "accessor " + std::to_string(idx) + ": byte offsets, lengths and stride must be non-negative integers";
       return false;
     }
-    a->base = buffers[(size_t)buf].data() + off;
+    const size_t size = buffers[(size_t)buf].size();
+    if (bv_off > size || (!bv["byteLength"].is_null() && bv_len > size - bv_off)) {
+      err = "bufferView " + std::to_string(bv_idx) + ": reads past the end of its buffer";
+      return false;
+    }
+    if (bv["byteLength"].is_null()) bv_len = size - bv_off;
+    const size_t elem = (size_t)cb * (size_t)a->ncomp;
+    if (a->stride == 0) a->stride = elem;
+    // off + stride * (count - 1) + elem <= the view's length, without overflow
+    if (a->count > 0 && (off > bv_len || elem > bv_len - off || a->count - 1 > (bv_len - off - elem) / a->stride)) {
+      err = "accessor " + std::to_string(idx) + ": reads past the end of its buffer view";
+      return false;
+    }
+    a->base = buffers[(size_t)buf].data() + bv_off + off;
     return true;
   }
 
@@ -271,9 +299,10 @@ struct Loader {
     if (bv_idx >= 0) {
       const JsonValue& bv = doc["bufferViews"][(size_t)bv_idx];
       const int buf = bv["buffer"].as_int(-1);
-      const size_t off = (size_t)bv["byteOffset"].as_number(0.0);
-      const size_t len = (size_t)bv["byteLength"].as_number(0.0);
-      if (buf < 0 || (size_t)buf >= buffers.size() || off + len > buffers[(size_t)buf].size()) {
+      size_t off = 0, len = 0;
+      if (buf < 0 || (size_t)buf >= buffers.size() || !json_size(bv["byteOffset"], &off) ||
+          !json_size(bv["byteLength"], &len) || off > buffers[(size_t)buf].size() ||
+          len > buffers[(size_t)buf].size() - off) {
         err = "image " + std::to_string(idx) + ": bad bufferView";
         return false;
       }
@@ -487,8 +516,18 @@ struct Loader {
         if (prim["mode"].as_int(4) != 4) continue;
         Accessor pa, ia;
         if (!accessor(prim["attributes"]["POSITION"].as_int(-1), &pa)) return false;
+        if (pa.ctype != 5126 || pa.ncomp != 3) {
+          err = "hitbox mesh '" + name + "': POSITION must be float VEC3";
+          return false;
+        }
         const int ii = prim["indices"].as_int(-1);
-        if (ii >= 0 && !accessor(ii, &ia)) return false;
+        if (ii >= 0) {
+          if (!accessor(ii, &ia)) return false;
+          if (ia.ncomp != 1 || (ia.ctype != 5121 && ia.ctype != 5123 && ia.ctype != 5125)) {
+            err = "hitbox mesh '" + name + "': indices must be unsigned scalars";
+            return false;
+          }
+        }
         const size_t ntri = ii >= 0 ? ia.count / 3 : pa.count / 3;
         for (size_t t = 0; t < ntri; ++t)
           for (int v = 0; v < 3; ++v) {

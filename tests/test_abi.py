@@ -215,6 +215,51 @@ def test_parse_errors(hostlib, tmp_path):
     assert "index out of range" in er.last_error(hostlib)
 
 
+def _parse_error(hostlib, path):
+    assert hostlib.parseGlTFTriangles(path.encode(), None, None, None, 0) == -1
+    return er.last_error(hostlib)
+
+
+@pytest.mark.parametrize("hitbox", [False, True])
+def test_parse_rejects_malformed_accessors(hostlib, tmp_path, hitbox):
+    """Untrusted accessor fields are checked before any read, on the hitbox
+    path as on the mesh path: POSITION must be float VEC3 and indices unsigned
+    scalars; counts, offsets and strides must be non-negative integers; a
+    count or stride that would read past the buffer is refused even where
+    offset + stride * (count - 1) wraps around 2^64."""
+    extras = {"hitbox": "True"} if hitbox else None
+    tag = "hb" if hitbox else "mesh"
+
+    def doc_with(edit):
+        doc, blob = quad_doc(POS, IDX, 5125, extras=extras)
+        edit(doc)
+        return write_doc(tmp_path, f"{tag}{len(list(tmp_path.iterdir()))}", doc, blob)
+
+    def pos_acc(**kw):
+        return lambda d: d["accessors"][0].update(kw)
+
+    assert "POSITION must be float VEC3" in _parse_error(hostlib, doc_with(pos_acc(type="VEC2")))
+    assert "POSITION must be float VEC3" in _parse_error(hostlib, doc_with(pos_acc(type="SCALAR")))
+    assert "POSITION must be float VEC3" in _parse_error(hostlib, doc_with(pos_acc(componentType=5123)))
+    assert "indices must be unsigned" in _parse_error(hostlib, doc_with(lambda d: d["accessors"][1].update(type="VEC2", count=3)))
+    assert "indices must be unsigned" in _parse_error(
+        hostlib, doc_with(lambda d: d["accessors"][1].update(componentType=5126)))
+    for bad in (-1, 2.5, 1e300, float(2 ** 60)):
+        assert "count must be" in _parse_error(hostlib, doc_with(pos_acc(count=bad))), bad
+        assert "non-negative integers" in _parse_error(hostlib, doc_with(pos_acc(byteOffset=bad))), bad
+        assert "non-negative integers" in _parse_error(
+            hostlib, doc_with(lambda d, b=bad: d["bufferViews"][0].update(byteStride=b))), bad
+    # in range for the JSON checks, but past the buffer (and, unchecked, wrapping)
+    assert "past the end" in _parse_error(hostlib, doc_with(pos_acc(count=5)))
+    assert "past the end" in _parse_error(hostlib, doc_with(pos_acc(count=2 ** 40)))
+    assert "past the end" in _parse_error(
+        hostlib, doc_with(lambda d: d["bufferViews"][0].update(byteStride=2 ** 47)))
+    assert "past the end" in _parse_error(hostlib, doc_with(pos_acc(byteOffset=2 ** 47)))
+    # the well-formed document still parses
+    pos, _, _ = er.parse_gltf_np(hostlib, doc_with(lambda d: None))
+    assert pos.shape[0] == (0 if hitbox else 2)
+
+
 def test_eye_file_helpers(tmp_path):
     omms = [er.Ommatidium([0.1, 0.2, 0.3], [0, 0, 1], 0.05, 0.0), er.Ommatidium([1, 2, 3], [1, 0, 0], 0.1, 0.01)]
     p = tmp_path / "x.eye"

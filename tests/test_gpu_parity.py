@@ -390,3 +390,35 @@ def test_spherical_projections(lib, config1, oracle_c1, shader, variant, ids):
     finally:
         lib.setCurrentEyeShaderName(b"single_dimension_fast")
         er.setRenderSize(lib, 400, 400)
+
+
+def test_batch_graph_rebuilt_with_scene(lib, config1, oracle_c1, tmp_path):
+    """The captured batch graph belongs to one scene build.  Reloading the same
+    triangles under the default background (same sizes, so the allocator can
+    hand back the same addresses) and rebuilding with another leaf size
+    between batches give each new scene's oracle colours: the graph is
+    captured again, with the new sky mode and the new BVH's stack bound."""
+    omm = config1["eyes"][0][:300]
+    poses6 = scenes.batch_poses(config1["scene"], 6, seed=12, extent=10.0)
+    poses12 = np.stack([oracle.pose12(p) for p in poses6])
+    setup_eye(lib, config1, omm=omm, spp=16, record=False)
+    _, rgb = er.render_batch_np(lib, poses6)
+    frame = lib.getCurrentEyeFrameIndex()
+    assert np.array_equal(rgb, oracle_c1.render(omm, poses12, spp=16, frame=frame)["colour_gpu_order"])
+    sc = config1["scene"]
+    dflt = scenes.Scene(sc.meshes, eyes=sc.eyes, background="default_background", heightfield=sc.heightfield)
+    path = scenes.write_gltf(tmp_path, "config1_default_bg", dflt)
+    setup_eye(lib, {"gltf": path, "pose": config1["pose"]}, omm=omm, spp=16, record=False)
+    _, rgb0 = er.render_batch_np(lib, poses6)
+    o0 = oracle.OracleScene(sc.tri_pos(), sc.tri_col(), sc.tri_mode(), 0, 4)
+    want0 = o0.render(omm, poses12, spp=16, frame=lib.getCurrentEyeFrameIndex())["colour_gpu_order"]
+    assert not np.array_equal(want0, rgb)
+    assert np.array_equal(rgb0, want0)
+    try:
+        assert lib.setSceneLeafSize(1) == 0
+        _, rgb1 = er.render_batch_np(lib, poses6)
+        o1 = oracle.OracleScene(sc.tri_pos(), sc.tri_col(), sc.tri_mode(), 0, 1)
+        assert er.bvh_info(lib)["stack_bound"] == o1.info()["stack_bound"]
+        assert np.array_equal(rgb1, want0)
+    finally:
+        assert lib.setSceneLeafSize(4) == 0

@@ -281,6 +281,56 @@ def test_shading_barycentric_and_sky():
     assert np.allclose(up, upper, atol=1e-7)
 
 
+def test_default_background_closed_forms():
+    """__miss__default_background (shaders.cu:740-747): r = (atan2(z, x) + pi)
+    / 2pi and g = (asin(y) + pi/2) / pi of the normalised direction, b = 0,
+    and black where any |component| < 0.01.  The diagonal has the closed form
+    (5/8, 1/2 + asin(1/sqrt 3)/pi); directions in every octant (none symmetric
+    in x and z, so swapped atan2 arguments show) match float64 numpy; the
+    border bands are black on either side of each axis plane and colour just
+    outside them."""
+    empty = oracle.OracleScene(np.zeros((0, 9), np.float32), np.zeros((0, 9), np.float32), np.zeros(0, np.uint32), 0)
+
+    def miss(dirs):
+        d = np.asarray(dirs, np.float32).reshape(-1, 3)
+        rays = np.zeros((len(d), 8), np.float32)
+        rays[:, 4:7] = d
+        rays[:, 7] = 1e16
+        return empty.radiance(rays)
+
+    diag = miss([1, 1, 1])[0]
+    assert abs(diag[0] - 0.625) <= 2e-7
+    assert abs(diag[1] - (0.5 + math.asin(1 / math.sqrt(3)) / math.pi)) <= 2e-7
+    assert diag[1] == pytest.approx(0.69591, abs=1e-5)
+    assert diag[2] == 0.0
+    dirs = np.float32([[2, 1, -3], [-1, 0.5, -2], [-3, -1, 0.5], [0.3, -2, 0.7], [-0.2, 3, 0.9],
+                       [0.7, -0.4, -0.1], [-1, -1, -3], [5, 0.2, 0.1]])
+    got = miss(dirs)
+    d = dirs.astype(np.float64)
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    want = np.stack([(np.arctan2(d[:, 2], d[:, 0]) + np.pi) / (2 * np.pi),
+                     (np.arcsin(d[:, 1]) + np.pi / 2) / np.pi, np.zeros(len(d))], 1)
+    assert np.all(np.abs(got - want) <= 2e-6), np.abs(got - want).max()
+    # the -x half plane: z -> +0 and z -> -0 give the two ends of the r range
+    hi, lo = miss([[-1, 0.5, 0.02], [-1, 0.5, -0.02]])
+    assert hi[0] > 0.99 and lo[0] < 0.01
+    for axis in range(3):
+        for v in (0.0, 0.009, -0.009, 0.0099):
+            dd = np.ones(3, np.float32)
+            dd[axis] = v
+            dd[(axis + 1) % 3] = -1.0
+            assert np.all(miss(dd)[0] == 0.0), (axis, v)
+        dd = np.ones(3, np.float32)
+        dd[axis] = 0.02   # 0.02 / |(0.02, 1, 1)| = 0.01414 > 0.01
+        assert np.all(miss(dd)[0][:2] > 0.0), axis
+    # the simple_sky mode of the same scene differs, so the mode switch is live
+    simple = oracle.OracleScene(np.zeros((0, 9), np.float32), np.zeros((0, 9), np.float32), np.zeros(0, np.uint32), 1)
+    rays = np.zeros((1, 8), np.float32)
+    rays[0, 4:7] = (1, 1, 1)
+    rays[0, 7] = 1e16
+    assert not np.array_equal(simple.radiance(rays)[0], diag)
+
+
 def test_material_colour_exact_and_sum_order():
     """Base-colour triangles are returned unchanged (no linearize).  With 64
     samples the pre-scaled values c/64 are exact and the wave-order sum only
