@@ -1,13 +1,23 @@
 """Benchmark: compound-eye rendering on one MI355X.
 
-Workload (the bee-scale plain-method configuration): two 5,500-ommatidium
-ellipsoidal-patch eyes (160 x 120 deg, acceptance U[1.5, 3] deg) at 128
-samples per ommatidium over a 2,189,058-triangle scene (1024^2 fBm heightfield
-+ 2,000 16-segment cylinders), generated in-run from fixed seeds and loaded
-through loadGlTFscene.  One step renders both eyes once (2 eye-frames,
-1,408,000 rays) through renderFrame; inputs are resident in HBM before the
-timed region.  The metric is eye-frames/s; Grays/s, the trace kernel's
-roofline fraction and a CPU-oracle baseline are reported beside it.
+Headline workload (the large heterogeneous batch): 256 eyes with seeded
+ommatidium counts log-uniform in [1k, 30k] (2,149,386 ommatidia), alternating
+Fibonacci-sphere and ellipsoidal-patch layouts, acceptance 1-5 deg, each at
+its own seeded pose over a 16,316,418-triangle scene (2048^2 heightfield +
+20,000 64-segment cylinders + 200 icosphere rocks), generated in-run from
+fixed seeds and loaded through loadGlTFscene.  The eyes' ommatidia are moved
+to world space and rendered as one batch.  One step is one learned adaptive
+render of the whole batch (8-sample pilot, sampler network, Neyman
+allocation, top-up: 256 eye-frames) at <= 1 % RMS colour error against a
+4,096-sample render; inputs are resident in HBM before the timed region.
+The metric is eye-frames/s; Grays/s, the gain over uniform sampling (and over
+the same allocation without the network) at the same error, the render
+kernel's roofline fraction and a CPU-oracle baseline are reported beside it.
+
+Extra keys: the bee-scale plain method through renderFrame (2 eyes x 5,500
+ommatidia x 128 spp, 2,189,058 triangles), learned adaptive sampling on the
+bee-scale scene (1,024 poses), and the 4,096-pose batch on the
+19,602-triangle heightfield.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -37,14 +47,14 @@ HBM_PEAK_GBS = 8000.0
 PROFILES = ROOT / "profiles"
 
 
-def measured_traffic(kernel: str):
-    """HBM bytes per launch of `kernel` on the bee-scale workload, from the
-    committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary named for that kernel
+def measured_traffic(kernel: str, workload: str):
+    """HBM bytes per launch of `kernel` on `workload` (c2, c5), from the
+    committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary named for both
     (already scaled by the copy-kernel factors recorded there); None when no
-    summary of this kernel is present."""
+    such summary is present."""
     base = kernel.split("<")[0]
     try:
-        for line in (PROFILES / f"{base}_c2_traffic.txt").read_text().splitlines():
+        for line in (PROFILES / f"{base}_{workload}_traffic.txt").read_text().splitlines():
             if line.startswith("traffic_bytes_per_launch:"):
                 return float(line.split(":", 1)[1])
     except (OSError, ValueError):
@@ -255,6 +265,10 @@ def bench_adaptive(lib, er, scenes, cfg, eyes, target: float = 0.01, n_test: int
             units += rgb.shape[0] * rgb.shape[1]
         return rms_abs(np.stack(got), np.stack(ref)), spp_sum / units
 
+    # the same allocation without the network (pooled pilot deviation)
+    lib.setAdaptiveSampler(1)
+    scale_nf, e_nf, spp_nf_ref = calibrate_budget(adaptive_err, target)
+    lib.setAdaptiveSampler(0)
     scale, e_a, spp_a_ref = calibrate_budget(adaptive_err, target)
 
     # 4. timed: 1,024 poses x both eyes, adaptive vs uniform batched vs renderFrame
@@ -274,6 +288,17 @@ def bench_adaptive(lib, er, scenes, cfg, eyes, target: float = 0.01, n_test: int
             lib.setCurrentEyeFrameIndex(0)
             total_a[0] += er.render_adaptive_np(lib, test, error_target=target, max_spp=1024, budget_scale=scale,
                                                 want_rgb=False)[3]
+
+    def run_network_free():
+        lib.setAdaptiveSampler(1)
+        try:
+            for i in eyes:
+                lib.gotoCamera(i)
+                lib.setCurrentEyeFrameIndex(0)
+                er.render_adaptive_np(lib, test, error_target=target, max_spp=1024, budget_scale=scale_nf,
+                                      want_rgb=False)
+        finally:
+            lib.setAdaptiveSampler(0)
 
     def run_uniform():
         for i in eyes:
@@ -295,6 +320,7 @@ def bench_adaptive(lib, er, scenes, cfg, eyes, target: float = 0.01, n_test: int
     frames = n_test * len(eyes)
     t_a = timed(run_adaptive)
     t_u = timed(run_uniform)
+    t_nf = timed(run_network_free)
     t_p = timed(run_plain, repeats=1)
     n_omm_total = sum(per_eye(lambda: lib.getCurrentEyeOmmatidialCount()))
     efs_a = frames / t_a
@@ -308,6 +334,11 @@ def bench_adaptive(lib, er, scenes, cfg, eyes, target: float = 0.01, n_test: int
         "eye_frames_per_s_uniform_renderframe": efs_p,
         "gain_vs_uniform_batched": efs_a / efs_u,
         "gain_vs_uniform_renderframe": efs_a / efs_p,
+        "eye_frames_per_s_network_free": frames / t_nf,
+        "gain_vs_network_free": t_nf / t_a,
+        "rms_abs_network_free": e_nf,
+        "network_free_mean_spp_ref_poses": spp_nf_ref,
+        "adaptive_mean_spp_ref_poses": spp_a_ref,
         "uniform_spp": spp_u,
         "adaptive_mean_spp": total_a[0] / (n_test * n_omm_total),
         "rms_abs_uniform": e_u,
@@ -321,12 +352,34 @@ def bench_adaptive(lib, er, scenes, cfg, eyes, target: float = 0.01, n_test: int
     }
 
 
-def bench_large(lib, er, scenes, tmp: str, target: float = 0.01, n_ref: int = 100_000) -> dict:
-    """Large heterogeneous batch: 256 eyes (log-uniform 1k-30k ommatidia,
-    Fibonacci-sphere and ellipsoidal layouts, acceptance 1-5 deg) at 256 poses
-    over a 13.4M-triangle scene, rendered as one batch (all eyes' ommatidia
-    moved to world space and traced in one persistent launch), learned
-    adaptive sampling against uniform sampling at the same RMS error."""
+def cpu_model() -> str:
+    name = platform.processor() or "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return name
+
+
+def traversal_bytes(st: dict, units: int, per_unit: int) -> float:
+    """Algorithmic bytes of traced rays: BVH nodes visited x 128 B + triangles
+    tested x 48 B + shading fetch (3 x 16 B) per hit, plus `per_unit` bytes of
+    per-unit records (ommatidium read, colour and statistics written)."""
+    return st["nodes_visited"] * 128.0 + st["tris_tested"] * 48.0 + st["hits"] * 48.0 + units * float(per_unit)
+
+
+def bench_large(lib, er, scenes, tmp: str, steps: int, warmup: int, target: float = 0.01,
+                ref_per_eye: int = 400, cpu_per_eye: int = 20, cpu: bool = True) -> dict:
+    """The headline: 256 eyes (log-uniform 1k-30k ommatidia, Fibonacci-sphere
+    and ellipsoidal layouts, acceptance 1-5 deg), each at its own pose over a
+    16.3M-triangle scene, rendered as one batch (all eyes' ommatidia moved to
+    world space and traced in one launch sequence).  The sampler network is
+    trained on a held-out batch; the error is measured on ref_per_eye seeded
+    random ommatidia of every eye against a 4,096-sample render, and on the
+    first cpu_per_eye of them against the CPU oracle's FP64-accumulated
+    4,096-sample render, which is also the timed CPU baseline."""
     t0 = time.perf_counter()
     scene = scenes.config5_scene()
     eyes = scenes.config5_eyes()
@@ -339,15 +392,18 @@ def bench_large(lib, er, scenes, tmp: str, target: float = 0.01, n_ref: int = 10
         raise RuntimeError(er.last_error(lib))
     bvh = er.bvh_info(lib)
     load_s = time.perf_counter() - t0
-    log(f"C5: {scene.tri_count()} triangles generated in {gen_s:.1f} s, loaded + BVH in {load_s:.1f} s")
+    log(f"C5: {scene.tri_count()} triangles generated in {gen_s:.1f} s, loaded + BVH in {load_s:.1f} s "
+        f"(build {bvh['build_ms']:.1f} ms)")
     er.gotoFirstCompoundEye(lib)
+    lib.setSampleRecording(False)
+    sizes = np.array([len(e) for e in eyes])
 
     def world_batch(seed):
         poses = scenes.batch_poses(scene, len(eyes), seed=seed, extent=80.0)
         return np.concatenate([scenes.world_ommatidia(e, scenes.pose_matrix12(p)) for e, p in zip(eyes, poses)])
 
     identity = np.zeros((1, 6), np.float32)
-    # training on held-out poses
+    # 1. training on a held-out batch (other poses), a quarter of its units
     t0 = time.perf_counter()
     train = world_batch(seed=901)
     er.set_ommatidia_np(lib, train)
@@ -360,16 +416,27 @@ def bench_large(lib, er, scenes, tmp: str, target: float = 0.01, n_ref: int = 10
         raise RuntimeError(er.last_error(lib))
     losses = er.mlp_train_np(lib, F, L, epochs=6, batch=4096, lr=2e-3, seed=6)
     train_s = time.perf_counter() - t0
-    log(f"C5: trained on {len(L)} rows in {train_s:.1f} s")
-    # the test batch; reference colours of its first n_ref ommatidia at 4,096 spp
+    log(f"C5: trained on {len(L)} rows in {train_s:.1f} s, loss {losses[0]:.3f} -> {losses[-1]:.3f}")
+    # 2. the test batch and its error sample: ref_per_eye random ommatidia of
+    # every eye, the first cpu_per_eye of each eye placed first
     batch = world_batch(seed=902)
     n_omm = len(batch)
-    er.set_ommatidia_np(lib, batch[:n_ref])
+    first = np.concatenate([[0], np.cumsum(sizes)])
+    rng = np.random.default_rng(903)
+    picks = [first[k] + rng.choice(sizes[k], size=min(ref_per_eye, sizes[k]), replace=False)
+             for k in range(len(eyes))]
+    ref_idx = np.concatenate([pk[:cpu_per_eye] for pk in picks] + [pk[cpu_per_eye:] for pk in picks])
+    n_cpu = cpu_per_eye * len(eyes)
+    er.set_ommatidia_np(lib, batch[ref_idx])
+    lib.setCameraPose(0.0, 0.0, 0.0, 0.0, 0.0, 0.0)
+    ref_eye = lib.getCurrentEyeIndex()
+    ref_frame = 1 << 20
     lib.setCurrentEyeSamplesPerOmmatidium(4096)
-    lib.setCurrentEyeFrameIndex(1 << 20)
+    lib.setCurrentEyeFrameIndex(ref_frame)
     ref = er.render_batch_np(lib, identity)[1][0]
     er.set_ommatidia_np(lib, batch)
     lib.setCameraPose(0.0, 0.0, 0.0, 0.0, 0.0, 0.0)
+    pose12 = er.pose_np(lib)
 
     def uniform(spp, want_rgb=True):
         lib.setCurrentEyeSamplesPerOmmatidium(spp)
@@ -377,16 +444,19 @@ def bench_large(lib, er, scenes, tmp: str, target: float = 0.01, n_ref: int = 10
         return er.render_batch_np(lib, identity, want_rgb=want_rgb)
 
     def err_of(rgb):
-        return rms_abs(rgb[0, :n_ref], ref)
+        return rms_abs(rgb[0, ref_idx], ref)
 
+    # 3. the uniform rate reaching the target (error ~ 1/sqrt(spp))
     e64 = err_of(uniform(64)[1])
     spp_u = max(1, int(math.ceil(64 * (e64 / target) ** 2)))
-    e_u = err_of(uniform(spp_u)[1])
+    rgb_u = uniform(spp_u)[1]
+    e_u = err_of(rgb_u)
     while e_u > target and spp_u < 4096:
         spp_u = int(math.ceil(spp_u * 1.1))
-        e_u = err_of(uniform(spp_u)[1])
-
-    # launch shape of both timed renders, autotuned on the uniform batch
+        rgb_u = uniform(spp_u)[1]
+        e_u = err_of(rgb_u)
+    # launch shape of every timed render, autotuned on the uniform batch (the
+    # adaptive passes keep its traversal and blocks and choose their grab size)
     lib.setCurrentEyeSamplesPerOmmatidium(spp_u)
     er.autotune_np(lib, identity, repeats=1)
     shape = er.render_tuning(lib)
@@ -396,11 +466,29 @@ def bench_large(lib, er, scenes, tmp: str, target: float = 0.01, n_ref: int = 10
         return er.render_adaptive_np(lib, identity, error_target=target, max_spp=512, budget_scale=scale,
                                      want_rgb=want_rgb)
 
-    def adaptive_err(scale):
-        _, rgb, _, total = adaptive(scale)
-        return err_of(rgb), total
+    def calibrated(sampler):
+        if lib.setAdaptiveSampler(sampler) != 0:
+            raise RuntimeError(er.last_error(lib))
 
-    scale, e_a, total = calibrate_budget(adaptive_err, target)
+        def err_at(scale):
+            _, rgb, _, total = adaptive(scale)
+            return err_of(rgb), total
+        scale, e, total = calibrate_budget(err_at, target)
+        _, rgb, counts, _ = adaptive(scale)
+        return scale, e, total, rgb, counts
+
+    scale_nf, e_nf, total_nf, rgb_nf, _ = calibrated(1)
+    scale, e_a, total, rgb_a, counts_a = calibrated(0)
+    # 4. the timed headline: warmup + steps learned adaptive renders of the batch
+    for _ in range(warmup):
+        adaptive(scale, want_rgb=False)
+    stage = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        adaptive(scale, want_rgb=False)
+        stage.append(er.adaptive_timings_np(lib))
+    elapsed = time.perf_counter() - t0
+    stage = np.array(stage)
 
     def timed(fn, repeats=3):
         fn()
@@ -409,51 +497,217 @@ def bench_large(lib, er, scenes, tmp: str, target: float = 0.01, n_ref: int = 10
             fn()
         return (time.perf_counter() - t) / repeats
 
-    t_a = timed(lambda: adaptive(scale, want_rgb=False))
     t_u = timed(lambda: uniform(spp_u, want_rgb=False))
+    lib.setAdaptiveSampler(1)
+    t_nf = timed(lambda: adaptive(scale_nf, want_rgb=False))
+    lib.setAdaptiveSampler(0)
     frames = len(eyes)
-    return {
-        "workload": f"{len(eyes)} eyes ({n_omm} ommatidia, log-uniform 1k-30k per eye) at {len(eyes)} poses, "
+    t_a = elapsed / steps
+    # 5. roofline of the render kernel: the exact traversal counts of the rays
+    # the two k_render_units launches trace (pilot: samples 0-7 of every unit;
+    # top-up: samples 8 .. n-1), counted by the per-ray statistics pass
+    er.set_sample_counts(lib, counts_a[0])
+    lib.setCurrentEyeFrameIndex(0)
+    st_all = er.traversal_stats(lib)
+    er.set_sample_counts(lib, None)
+    lib.setCurrentEyeSamplesPerOmmatidium(8)
+    lib.setCurrentEyeFrameIndex(0)
+    st_pilot = er.traversal_stats(lib)
+    st_top = {k: st_all[k] - st_pilot[k] for k in st_all}
+    # pilot: ommatidium 32 B read, colour 16 B + statistics 32 B written per
+    # unit; top-up: ommatidium 32 B + count 4 B read, colour 16 B written
+    b_pilot = traversal_bytes(st_pilot, n_omm, 32 + 16 + 32)
+    b_top = traversal_bytes(st_top, n_omm, 32 + 4 + 16)
+    ms_pilot = float(stage[:, 0].mean())
+    ms_top = float(stage[:, 2].mean())
+    launch_ms = (ms_pilot + ms_top) / 2
+    bytes_launch = (b_pilot + b_top) / 2
+    achieved = bytes_launch / (launch_ms * 1e-3) / 1e9
+    traffic = measured_traffic("k_render_units", "c5")
+    out = {
+        "workload": f"{len(eyes)} eyes ({n_omm} ommatidia, log-uniform 1k-30k per eye), each at its own pose, "
                     f"{scene.tri_count()} triangles (2048^2 heightfield + 20,000 cylinders + 200 rocks), "
                     "one batch per render",
         "eye_frames_per_s_adaptive": frames / t_a,
         "eye_frames_per_s_uniform_batched": frames / t_u,
+        "eye_frames_per_s_network_free": frames / t_nf,
         "gain_vs_uniform_batched": t_u / t_a,
+        "gain_vs_network_free": t_nf / t_a,
         "grays_per_s_adaptive": total / t_a / 1e9,
         "grays_per_s_uniform": n_omm * spp_u / t_u / 1e9,
         "uniform_spp": spp_u,
         "adaptive_mean_spp": total / n_omm,
+        "network_free_mean_spp": total_nf / n_omm,
         "rms_abs_uniform": e_u,
         "rms_abs_adaptive": e_a,
-        "rms_reference": f"first {n_ref} ommatidia of the batch, uniform 4,096-spp render on the GPU",
+        "rms_abs_network_free": e_nf,
+        "rms_rel_adaptive": e_a / float(np.sqrt(np.mean(ref.astype(np.float64) ** 2))),
+        "rms_reference": f"{len(ref_idx)} ommatidia ({ref_per_eye} seeded random per eye, all 256 eyes), "
+                         "4,096-spp render on the GPU",
         "budget_scale": scale,
+        "budget_scale_network_free": scale_nf,
+        "stage_ms": {"pilot": ms_pilot, "allocation": float(stage[:, 1].mean()), "top_up": ms_top,
+                     "combine": float(stage[:, 3].mean())},
+        "traversal_per_ray": {"nodes": st_all["nodes_visited"] / total, "tris": st_all["tris_tested"] / total,
+                              "hit_fraction": st_all["hits"] / total},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": "k_render_units",
+                     "bytes_per_launch": bytes_launch, "launch_ms": launch_ms,
+                     "launches": "pilot and top-up of one adaptive render (HIP events on the library stream)",
+                     "hbm_frac_measured": None if traffic is None else traffic / (launch_ms * 1e-3) / 1e9 /
+                     HBM_PEAK_GBS},
         "launch_shape": shape,
         "bvh": bvh,
         "scene_load_s": load_s,
         "training": {"rows": int(len(L)), "label_spp": 1024, "seconds": train_s, "loss_first": float(losses[0]),
                      "loss_last": float(losses[-1])},
+        "_timed": {"elapsed": elapsed, "steps": steps, "frames_per_step": frames, "samples_per_step": total},
     }
+    if cpu:
+        import oracle  # the CPU oracle: timed here as the CPU baseline, and the FP64 reference
+        t0 = time.perf_counter()
+        osc = oracle.OracleScene.from_scene(scene)
+        build_s = time.perf_counter() - t0
+        sample = batch[ref_idx[:n_cpu]]
+        threads = oracle.default_threads()
+        t0 = time.perf_counter()
+        r = osc.render(sample, pose12, spp=4096, eye=ref_eye, frame=ref_frame, want_f64=True, threads=threads)
+        dt = time.perf_counter() - t0
+        rays = n_cpu * 4096
+        f64 = r["colour_f64"][0]
+        sel = ref_idx[:n_cpu]
+        # the GPU's 4,096-sample reference renders the same rays in FP32
+        out["rms_vs_fp64_oracle"] = {
+            "ommatidia": n_cpu,
+            "adaptive": rms_abs(rgb_a[0, sel], f64),
+            "uniform": rms_abs(rgb_u[0, sel], f64),
+            "network_free": rms_abs(rgb_nf[0, sel], f64),
+            "gpu_4096_reference_max_abs_diff": float(np.max(np.abs(ref[:n_cpu].astype(np.float64) - f64))),
+            "gpu_4096_reference_bit_identical_to_oracle_fp32": bool(np.array_equal(ref[:n_cpu],
+                                                                                  r["colour_gpu_order"][0])),
+        }
+        rays_per_s = rays / dt
+        out["cpu_baseline"] = {
+            "value": rays_per_s / (n_omm / len(eyes) * spp_u),
+            "unit": "eye-frames/s",
+            "cores": threads,
+            "kind": "port",
+            "sample": f"{n_cpu} ommatidia ({cpu_per_eye} per eye, all 256 eyes) x 4096 spp ({rays} rays, "
+                      f"FP64 accumulation) in {dt:.2f} s = {rays_per_s / 1e6:.3f} Mrays/s, scaled to eye-frames "
+                      f"of the plain method at the same error ({n_omm / len(eyes):.0f} ommatidia per eye x "
+                      f"{spp_u} spp); oracle BVH build {build_s:.1f} s excluded; CPU {cpu_model()}",
+            "grays_per_s": rays_per_s / 1e9,
+        }
+    return out
+
+
+def bench_plain(lib, er, cfg: dict, eyes: list, steps: int, warmup: int, trace_mode: int, copy_probe: float,
+                cpu: bool, rms: bool) -> dict:
+    """The plain method at bee scale: both eyes through renderFrame at 128
+    spp, the reference's way of rendering (libEyeRenderer.cpp:229-242)."""
+    spp = cfg["spp"]
+    rays_per_step = 0
+    stats = {"nodes_visited": 0, "tris_tested": 0, "hits": 0}
+    omm_total = 0
+    for i in eyes:
+        lib.gotoCamera(i)
+        lib.setCurrentEyeSamplesPerOmmatidium(spp)
+        lib.setCameraPose(*[float(v) for v in cfg["pose"]])
+        rays_per_step += int(lib.getCurrentEyeTotalSamples())
+        omm_total += int(lib.getCurrentEyeOmmatidialCount())
+        s = er.traversal_stats(lib)
+        for k in stats:
+            stats[k] += s[k]
+
+    def step():
+        for i in eyes:
+            lib.gotoCamera(i)
+            if lib.renderFrame() < 0:
+                raise RuntimeError(er.last_error(lib))
+
+    for _ in range(warmup):
+        step()
+    trace_ms = []
+    kernel_name = ""
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        for i in eyes:
+            lib.gotoCamera(i)
+            if lib.renderFrame() < 0:
+                raise RuntimeError(er.last_error(lib))
+            trace_ms.append(er.timings_np(lib)[0])
+            kernel_name = lib.getLastRenderKernel().decode()
+    elapsed = time.perf_counter() - t0
+    trace_avg_ms = float(np.mean(trace_ms))
+    # algorithmic bytes of one render launch (one eye-frame), per ray as the
+    # rays' own traversals count them (a packet fetches a node once per wave)
+    bytes_launch = traversal_bytes(stats, omm_total, 32 + 16) / len(eyes)
+    achieved = bytes_launch / (trace_avg_ms * 1e-3) / 1e9
+    traffic = measured_traffic(kernel_name, "c2")
+    if copy_probe > 0:
+        nbytes = int(copy_probe * 2 ** 20)
+        cms = lib.runCopyProbe(nbytes, 10)
+        log(f"copy probe: {nbytes} B in {cms:.4f} ms = {2 * nbytes / cms / 1e6:.1f} GB/s (read + write)")
+    out = {
+        "workload": "plain method, bee-scale: 2 eyes x 5,500 ommatidia x 128 spp through renderFrame, "
+                    f"{cfg['scene'].tri_count()} triangles (1024^2 fBm heightfield + 2,000 cylinders)",
+        "eye_frames_per_s": steps * len(eyes) / elapsed,
+        "ms_per_step": elapsed / steps * 1e3,
+        "grays_per_s": steps * rays_per_step / elapsed / 1e9,
+        "rays_per_step": rays_per_step,
+        "trace_kernel_ms": trace_avg_ms,
+        "traversal_per_ray": {"nodes": stats["nodes_visited"] / rays_per_step,
+                              "tris": stats["tris_tested"] / rays_per_step,
+                              "hit_fraction": stats["hits"] / rays_per_step},
+        "roofline": {"bound": "latency of dependent node fetches (L2/MALL-resident scene)",
+                     "achieved_algorithmic": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac_algorithmic": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "hbm_frac_measured": None if traffic is None else traffic / (trace_avg_ms * 1e-3) / 1e9 /
+                     HBM_PEAK_GBS,
+                     "kernel": kernel_name, "bytes_per_launch": bytes_launch, "trace_mode": trace_mode},
+    }
+    if rms:
+        # RMS colour error of this render against a 4096-spp render of the same eye
+        lib.gotoCamera(eyes[0])
+        col128 = er.colours_np(lib)
+        lib.setCurrentEyeSamplesPerOmmatidium(4096)
+        lib.setCurrentEyeFrameIndex(1 << 20)
+        lib.renderFrame()
+        col4096 = er.colours_np(lib)
+        lib.setCurrentEyeSamplesPerOmmatidium(spp)
+        e = float(np.sqrt(np.mean((col128.astype(np.float64) - col4096) ** 2)))
+        out["rms_colour_err_vs_4096spp"] = {"abs": e, "rel": e / float(np.sqrt(np.mean(col4096.astype(np.float64) ** 2))),
+                                            "note": "eye 0, GPU 4096-spp render as reference"}
+    if cpu:
+        lib.gotoCamera(eyes[0])
+        out["cpu_baseline"] = cpu_baseline(cfg, spp, er.pose_np(lib), target_s=5.0)
+        out["gpu_vs_cpu"] = out["eye_frames_per_s"] / out["cpu_baseline"]["value"]
+    return out
 
 
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--profile", action="store_true",
-                    help="render loop only (no 4096-spp error check, no CPU baseline): for profiler runs")
+                    help="headline renders only (no CPU baseline, no other workloads): for profiler runs")
     ap.add_argument("--skip-batched", action="store_true", help="skip the 4,096-pose batched configuration")
-    ap.add_argument("--skip-adaptive", action="store_true", help="skip the learned adaptive sampling comparison")
+    ap.add_argument("--skip-adaptive", action="store_true", help="skip learned adaptive sampling at bee scale")
+    ap.add_argument("--skip-plain", action="store_true", help="skip the bee-scale renderFrame workload")
+    ap.add_argument("--skip-large", action="store_true",
+                    help="skip the headline large batch (the line then carries no headline value)")
     ap.add_argument("--adaptive-only", action="store_true",
-                    help="run only the learned adaptive comparison on the bee-scale scene (profiler runs)")
+                    help="run only learned adaptive sampling at bee scale (profiler runs)")
     ap.add_argument("--batched-only", action="store_true",
                     help="run only the 4,096-pose batched configuration (profiler runs of the persistent renderer)")
-    ap.add_argument("--large", action="store_true",
-                    help="also run the 256-eye, 13.4M-triangle heterogeneous batch (several minutes)")
+    ap.add_argument("--plain-only", action="store_true",
+                    help="run only the bee-scale renderFrame workload (profiler runs of k_render_tasks)")
     ap.add_argument("--no-split", action="store_true", help="disable split launches (chunks of a unit on several waves)")
     ap.add_argument("--trace-mode", type=int, default=1, choices=(0, 1, 2),
-                    help="traversal: 0 per-ray, 1 wave packets (default), 2 wave packets with scalar node loads")
+                    help="renderFrame traversal: 0 per-ray, 1 wave packets (default), 2 wave packets with scalar "
+                         "node loads")
     ap.add_argument("--copy-probe", type=float, default=0.0,
                     help="also run a plain copy kernel of this many MiB (profiler bandwidth reference)")
     args = ap.parse_args()
@@ -477,129 +731,82 @@ def main() -> int:
         lib.stop()
         print(json.dumps(out))
         return 0
-    t0 = time.time()
-    cfg = scenes.build_config(2, tmp)
-    log(f"generated C2 scene ({cfg['scene'].tri_count()} triangles) in {time.time() - t0:.1f} s")
-    spp = cfg["spp"]
-    t0 = time.time()
-    lib.loadGlTFscene(str(cfg["gltf"]).encode())
-    err = er.last_error(lib)
-    if err:
-        raise RuntimeError(err)
-    bvh = er.bvh_info(lib)
-    log(f"loaded + GPU LBVH in {time.time() - t0:.1f} s (build {bvh['build_ms']:.1f} ms, {bvh['nodes']} nodes)")
-    eyes = []
-    for i in range(lib.getCameraCount()):
-        lib.gotoCamera(i)
-        if lib.isCompoundEyeActive():
-            lib.setCurrentEyeSamplesPerOmmatidium(spp)
-            lib.setCameraPose(*[float(v) for v in cfg["pose"]])
-            eyes.append(i)
-    assert len(eyes) == 2
-    if args.adaptive_only:
-        out = {"learned_adaptive": bench_adaptive(lib, er, scenes, cfg, eyes)}
-        lib.stop()
-        print(json.dumps(out))
-        return 0
-    rays_per_step = 0
-    stats = {"nodes_visited": 0, "tris_tested": 0, "hits": 0}
-    omm_counts = []
-    for i in eyes:
-        lib.gotoCamera(i)
-        rays_per_step += int(lib.getCurrentEyeTotalSamples())
-        omm_counts.append(int(lib.getCurrentEyeOmmatidialCount()))
-        s = er.traversal_stats(lib)
-        for k in stats:
-            stats[k] += s[k]
-
-    def step():
-        for i in eyes:
-            lib.gotoCamera(i)
-            if lib.renderFrame() < 0:
-                raise RuntimeError(er.last_error(lib))
-
-    for _ in range(args.warmup):
-        step()
-    trace_ms = []
-    kernel_name = ""
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        for i in eyes:
-            lib.gotoCamera(i)
-            if lib.renderFrame() < 0:
-                raise RuntimeError(er.last_error(lib))
-            trace_ms.append(er.timings_np(lib)[0])
-            kernel_name = lib.getLastRenderKernel().decode()
-    elapsed = time.perf_counter() - t0
-    eye_frames = args.steps * len(eyes)
-    value = eye_frames / elapsed
-    grays = args.steps * rays_per_step / elapsed / 1e9
-    trace_avg_ms = float(np.mean(trace_ms))
-    # algorithmic bytes of one render launch (one eye-frame): BVH nodes visited
-    # x 128 B + triangles tested x 48 B + shading fetch (3 x 16 B per hit) +
-    # per ommatidium its 32 B record read and its 16 B colour written
-    omm_total = sum(omm_counts)
-    bytes_launch = (stats["nodes_visited"] * 128 + stats["tris_tested"] * 48 + stats["hits"] * 48
-                    + omm_total * (32 + 16)) / len(eyes)
-    achieved = bytes_launch / (trace_avg_ms * 1e-3) / 1e9
-    if args.copy_probe > 0:
-        nbytes = int(args.copy_probe * 2 ** 20)
-        cms = lib.runCopyProbe(nbytes, 10)
-        log(f"copy probe: {nbytes} B in {cms:.4f} ms = {2 * nbytes / cms / 1e6:.1f} GB/s (read + write)")
-    rms = rms_rel = None
-    if not args.profile:
-        # RMS colour error of this render against a 4096-spp render of the same eye
-        lib.gotoCamera(eyes[0])
-        col128 = er.colours_np(lib)
-        lib.setCurrentEyeSamplesPerOmmatidium(4096)
-        lib.setCurrentEyeFrameIndex(1 << 20)
-        lib.renderFrame()
-        col4096 = er.colours_np(lib)
-        lib.setCurrentEyeSamplesPerOmmatidium(spp)
-        rms = float(np.sqrt(np.mean((col128.astype(np.float64) - col4096) ** 2)))
-        rms_rel = rms / float(np.sqrt(np.mean(col4096.astype(np.float64) ** 2)))
+    only = args.adaptive_only or args.plain_only
     out = {
         "metric": METRIC,
-        "value": value,
+        "value": None,
         "unit": "eye-frames/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": elapsed / args.steps * 1e3,
+        "ms_per_step": None,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic: seeded C2 scene and eyes generated in-run, loaded via loadGlTFscene",
-        "config": {
-            "workload": "plain method, bee-scale: 2 eyes x 5,500 ommatidia x 128 spp, "
-                        f"{cfg['scene'].tri_count()} triangles (1024^2 fBm heightfield + 2,000 cylinders)",
-            "rays_per_step": rays_per_step,
-            "eye_frames_per_step": len(eyes),
-            "bvh": bvh,
-        },
-        "grays_per_s": grays,
-        "rms_colour_err_vs_4096spp": {"abs": rms, "rel": rms_rel, "note": "eye 0, GPU 4096-spp render as reference"},
-        "trace_kernel_ms": trace_avg_ms,
-        "traversal_per_ray": {"nodes": stats["nodes_visited"] / rays_per_step,
-                              "tris": stats["tris_tested"] / rays_per_step,
-                              "hit_fraction": stats["hits"] / rays_per_step},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(kernel_name),
-                     "kernel": kernel_name, "bytes_per_launch": bytes_launch,
-                     "trace_mode": args.trace_mode},
+        "data": "synthetic: seeded scenes and eyes generated in-run, loaded via loadGlTFscene; "
+                "sampler network trained in-run on held-out poses",
     }
-    if not (args.no_cpu_baseline or args.profile):
-        lib.gotoCamera(eyes[0])
-        pose12 = er.pose_np(lib)
-        out["cpu_baseline"] = cpu_baseline(cfg, spp, pose12)
-    if not args.profile and not args.skip_adaptive:
-        log("learned adaptive sampling ...")
-        out["learned_adaptive"] = bench_adaptive(lib, er, scenes, cfg, eyes)
-    if args.large:
-        log("large heterogeneous batch ...")
-        out["large_batch"] = bench_large(lib, er, scenes, tmp)
-    if not args.profile and not args.skip_batched:
+    if not (only or args.skip_large):
+        log("large heterogeneous batch (headline) ...")
+        large = bench_large(lib, er, scenes, tmp, args.steps, args.warmup,
+                            cpu=not (args.no_cpu_baseline or args.profile))
+        timed = large.pop("_timed")
+        out["value"] = timed["frames_per_step"] * timed["steps"] / timed["elapsed"]
+        out["ms_per_step"] = timed["elapsed"] / timed["steps"] * 1e3
+        out["config"] = {
+            "workload": "learned adaptive + batched, large heterogeneous batch: " + large["workload"]
+                        + f"; <= 0.01 RMS colour error; one step = one adaptive render of the batch "
+                          f"({timed['frames_per_step']} eye-frames)",
+            "eye_frames_per_step": timed["frames_per_step"],
+            "samples_per_step": timed["samples_per_step"],
+            "bvh": large["bvh"],
+        }
+        out["grays_per_s"] = large["grays_per_s_adaptive"]
+        out["gain_vs_uniform_batched"] = large["gain_vs_uniform_batched"]
+        out["gain_vs_network_free"] = large["gain_vs_network_free"]
+        out["rms_abs_adaptive"] = large["rms_abs_adaptive"]
+        out["roofline"] = large.pop("roofline")
+        if "cpu_baseline" in large:
+            out["cpu_baseline"] = large.pop("cpu_baseline")
+        out["large_batch"] = large
+        log(f"C5: {out['value']:.0f} eye-frames/s adaptive, gain {large['gain_vs_uniform_batched']:.2f}x over "
+            f"uniform, {large['gain_vs_network_free']:.2f}x over network-free")
+    if args.profile and not only:
+        lib.stop()
+        print(json.dumps(out))
+        return 0
+    want_plain = not args.skip_plain and not args.adaptive_only
+    want_adaptive = not args.skip_adaptive and not args.plain_only
+    if want_plain or want_adaptive:
+        t0 = time.time()
+        cfg = scenes.build_config(2, tmp)
+        log(f"generated C2 scene ({cfg['scene'].tri_count()} triangles) in {time.time() - t0:.1f} s")
+        t0 = time.time()
+        lib.loadGlTFscene(str(cfg["gltf"]).encode())
+        err = er.last_error(lib)
+        if err:
+            raise RuntimeError(err)
+        bvh = er.bvh_info(lib)
+        log(f"loaded + GPU LBVH in {time.time() - t0:.1f} s (build {bvh['build_ms']:.1f} ms, {bvh['nodes']} nodes)")
+        eyes = []
+        for i in range(lib.getCameraCount()):
+            lib.gotoCamera(i)
+            if lib.isCompoundEyeActive():
+                lib.setCurrentEyeSamplesPerOmmatidium(cfg["spp"])
+                lib.setCameraPose(*[float(v) for v in cfg["pose"]])
+                eyes.append(i)
+        assert len(eyes) == 2
+        if want_plain:
+            log("plain method at bee scale (renderFrame) ...")
+            plain = bench_plain(lib, er, cfg, eyes, args.steps, args.warmup, args.trace_mode, args.copy_probe,
+                                cpu=not (args.no_cpu_baseline or args.profile), rms=not args.profile)
+            plain["bvh"] = bvh
+            out["plain_bee_scale"] = plain
+        if want_adaptive:
+            log("learned adaptive sampling at bee scale ...")
+            out["learned_adaptive_bee_scale"] = bench_adaptive(lib, er, scenes, cfg, eyes)
+    if not only and not args.skip_batched:
         log("batched configuration ...")
         out["batched"] = bench_batched(lib, er, scenes, tmp)
     lib.stop()

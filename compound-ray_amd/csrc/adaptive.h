@@ -13,7 +13,8 @@
 //      absolute difference of their pilot mean RGB from the unit's, and the
 //      mean of their inverse-depth standard deviation;
 //   3. the MLP predicts log2 of the samples the unit needs for an RMS error of
-//      `error_target` on its own: n_req = mean RGB variance / error^2;
+//      `error_target` on its own: n_req = mean RGB variance / error^2 (or,
+//      without the network, n_req from the pooled pilot deviation, below);
 //   4. allocation minimising the total samples for a mean squared error of
 //      error_target^2 over the eye (Neyman): n_i = sqrt(n_req,i) * sum_j
 //      sqrt(n_req,j) / N, clamped to [pilot, max_spp];
@@ -41,11 +42,22 @@ struct AdaptiveConfig {
   int max_spp = 512;
   float budget_scale = 1.0f;   // multiplies every allocation (calibration knob)
   const uint32_t* neighbours = nullptr;  // device, kNeighbours per ommatidium (nearest_ommatidia)
+  int sampler = 0;             // AdaptiveSampler: where n_req comes from
 };
+
+// Source of each unit's required samples n_req: the trained network, or the
+// network-free estimate from the pooled pilot standard deviation of the unit
+// and its kNeighbours nearest ommatidia, sigma = (own + k * neighbours' mean)
+// / (k + 1), n_req = sigma^2 / label_error^2.  The allocation, the top-up and
+// the combination are the same for both.
+enum AdaptiveSampler : int { kSamplerNetwork = 0, kSamplerPooledSigma = 1 };
 
 struct AdaptiveResult {
   double gpu_ms = 0.0;
   uint64_t total_samples = 0;
+  // device milliseconds of the stages: pilot render, features + n_req +
+  // allocation, top-up render, combination
+  double stage_ms[4] = {0.0, 0.0, 0.0, 0.0};
 };
 
 class AdaptiveRenderer {
@@ -82,6 +94,7 @@ class AdaptiveRenderer {
   uint32_t* d_queue_ = nullptr;
   size_t units_cap_ = 0;
   hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
+  hipEvent_t ev_stage_[3] = {nullptr, nullptr, nullptr};   // after the pilot, the allocation, the top-up
 };
 
 }  // namespace eye

@@ -133,6 +133,19 @@ __global__ void k_labels(const float4* __restrict__ stats, uint64_t units, float
   lab[u] = fminf(log2f(n), 12.0f);
 }
 
+// Network-free n_req: log2 of the squared pooled pilot standard deviation
+// over label_error^2, from the unit's features (own RGB deviations 7-9 and the
+// neighbours' mean deviation 12).
+__global__ void k_pooled_sigma(const float* __restrict__ feat, uint64_t units, float inv_err2,
+                               float* __restrict__ pred) {
+  const uint64_t u = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (u >= units) return;
+  const float* f = feat + u * kMlpIn;
+  const float own = (f[7] + f[8] + f[9]) * (1.0f / 3.0f);
+  const float sigma = (own + (float)kNeighbours * f[12]) * (1.0f / (float)(kNeighbours + 1));
+  pred[u] = log2f(fmaxf(sigma * sigma * inv_err2, 1.0f));
+}
+
 __global__ void k_zero_u64(unsigned long long* p) { *p = 0ull; }
 
 template <typename T>
@@ -172,10 +185,16 @@ void AdaptiveRenderer::release() {
   if (ev0_) (void)hipEventDestroy(ev0_);
   if (ev1_) (void)hipEventDestroy(ev1_);
   ev0_ = ev1_ = nullptr;
+  for (hipEvent_t& e : ev_stage_) {
+    if (e) (void)hipEventDestroy(e);
+    e = nullptr;
+  }
 }
 
 bool AdaptiveRenderer::ensure(size_t poses, size_t units, std::string* err) {
-  if (!ev0_ && (hipEventCreate(&ev0_) != hipSuccess || hipEventCreate(&ev1_) != hipSuccess)) {
+  if (!ev0_ && (hipEventCreate(&ev0_) != hipSuccess || hipEventCreate(&ev1_) != hipSuccess ||
+                hipEventCreate(&ev_stage_[0]) != hipSuccess || hipEventCreate(&ev_stage_[1]) != hipSuccess ||
+                hipEventCreate(&ev_stage_[2]) != hipSuccess)) {
     if (err) *err = "adaptive: event creation failed";
     return false;
   }
@@ -201,7 +220,11 @@ bool AdaptiveRenderer::render(const DeviceScene& scene, UnitLaunch U, const std:
   const size_t np = poses.size();
   const size_t units = np * (size_t)U.omm_count;
   if (units == 0) return true;
-  if (!mlp.ready()) {
+  if (cfg.sampler != kSamplerNetwork && cfg.sampler != kSamplerPooledSigma) {
+    if (err) *err = "adaptive: unknown sampler";
+    return false;
+  }
+  if (cfg.sampler == kSamplerNetwork && !mlp.ready()) {
     if (err) *err = "adaptive: the sampler network is not initialised (mlpInit / mlpTrain)";
     return false;
   }
@@ -235,17 +258,25 @@ bool AdaptiveRenderer::render(const DeviceScene& scene, UnitLaunch U, const std:
   // batches; a grab of few pilot units would leave most of the wave's lanes idle.
   A.units_per_grab = 0;
   if (!launch_render_units(scene, A, stream, err)) return false;
+  (void)hipEventRecord(ev_stage_[0], stream);
   const unsigned gb = (unsigned)((units + kThreads - 1) / kThreads);
   hipLaunchKernelGGL(k_features, dim3(gb), dim3(kThreads), 0, stream, U.omms, d_poses_, U.omm_count, (uint64_t)units,
                      d_col_a_, d_stats_, cfg.neighbours, d_feat_);
   if (!ok_launch("features", err)) return false;
-  if (!mlp.predict(d_feat_, (int64_t)units, d_pred_, stream, err)) return false;
+  if (cfg.sampler == kSamplerNetwork) {
+    if (!mlp.predict(d_feat_, (int64_t)units, d_pred_, stream, err)) return false;
+  } else {
+    hipLaunchKernelGGL(k_pooled_sigma, dim3(gb), dim3(kThreads), 0, stream, d_feat_, (uint64_t)units,
+                       1.0f / (cfg.label_error * cfg.label_error), d_pred_);
+    if (!ok_launch("pooled sigma", err)) return false;
+  }
   hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, stream, d_total_);
   hipLaunchKernelGGL(k_allocate, dim3((unsigned)np), dim3(kThreads), 0, stream, d_pred_, U.omm_count, cfg.pilot,
                      cfg.max_spp, cfg.budget_scale * (cfg.label_error / cfg.error_target) *
                                       (cfg.label_error / cfg.error_target),
                      d_counts_, d_extra_, d_total_);
   if (!ok_launch("allocate", err)) return false;
+  (void)hipEventRecord(ev_stage_[1], stream);
   UnitLaunch B = A;
   B.unit_counts = d_extra_;
   B.sample_base = (uint32_t)cfg.pilot;
@@ -255,6 +286,7 @@ bool AdaptiveRenderer::render(const DeviceScene& scene, UnitLaunch U, const std:
   // fewer, closer ommatidia (measured 1-2 % faster than 16 on the bee eye).
   B.auto_grab_max = 8;
   if (!launch_render_units(scene, B, stream, err)) return false;
+  (void)hipEventRecord(ev_stage_[2], stream);
   hipLaunchKernelGGL(k_combine, dim3(gb), dim3(kThreads), 0, stream, d_col_a_, d_col_b_, d_extra_, cfg.pilot,
                      (uint64_t)units, d_final_);
   if (!ok_launch("combine", err)) return false;
@@ -270,6 +302,12 @@ bool AdaptiveRenderer::render(const DeviceScene& scene, UnitLaunch U, const std:
   if (res) {
     res->gpu_ms = ms;
     res->total_samples = total;
+    const hipEvent_t marks[5] = {ev0_, ev_stage_[0], ev_stage_[1], ev_stage_[2], ev1_};
+    for (int k = 0; k < 4; ++k) {
+      float f = 0.0f;
+      (void)hipEventElapsedTime(&f, marks[k], marks[k + 1]);
+      res->stage_ms[k] = f;
+    }
   }
   if (rgb) {
     std::vector<float4> tmp(units);

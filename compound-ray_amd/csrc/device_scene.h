@@ -39,6 +39,9 @@ struct DeviceScene {
   // after each build): caches keyed on a scene compare it, because a rebuilt
   // scene can reuse the freed buffers' addresses.
   uint64_t generation = 0;
+  // Device address of a host-visible word (not owned) that a packet traversal
+  // entered by a partial wave sets; null: not reported.
+  uint32_t* fault = nullptr;
   float centroid_lo[3] = {0, 0, 0}, centroid_hi[3] = {0, 0, 0};
 };
 

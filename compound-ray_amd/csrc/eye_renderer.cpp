@@ -122,6 +122,8 @@ struct State {
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   double timings[4] = {0, 0, 0, 0};   // trace, reduce, projection, total GPU ms of the last render
   unsigned long long* d_stats = nullptr;
+  uint32_t* h_fault = nullptr;   // mapped host word: packet traversal entered by a partial wave
+  uint32_t* d_fault = nullptr;   // its device address
   BatchRenderer batch;
   Mlp mlp;
   AdaptiveRenderer adaptive;
@@ -131,6 +133,8 @@ struct State {
   int unit_blocks = 0;           // 0: automatic
   bool autotune = false;         // tune renderPoseBatch's launch shape per workload
   float label_error = 0.01f;     // error the sampler network's labels were made for
+  int sampler = kSamplerNetwork; // source of the adaptive renders' required samples
+  double adaptive_ms[4] = {0, 0, 0, 0};  // stage times of the last adaptive render
   LaunchTuner tuner;
   std::string tune_cache;        // file the tuner's entries are saved to (empty: none)
   std::string name_buf;
@@ -167,7 +171,23 @@ void ensure_hip() {
   hip_check(hipMalloc(&s.d_stats, sizeof(unsigned long long) * 4), "hipMalloc stats");
   hip_check(hipMalloc(&s.d_queue, sizeof(uint32_t) * kQueueWords), "hipMalloc queue");
   for (hipEvent_t& e : s.ev) hip_check(hipEventCreate(&e), "hipEventCreate");
+  hip_check(hipHostMalloc(&s.h_fault, sizeof(uint32_t), hipHostMallocMapped | hipHostMallocCoherent),
+            "hipHostMalloc fault word");
+  *s.h_fault = 0u;
+  hip_check(hipHostGetDevicePointer((void**)&s.d_fault, s.h_fault, 0), "fault word device address");
   s.hip_ready = true;
+}
+
+// The packet traversal flags a launch that entered it with a partial wave
+// and returns misses there: the call that rendered fails instead of handing
+// back those colours.  Called after the call's stream synchronisation.
+void check_packet_fault(const char* what) {
+  State& s = st();
+  if (s.h_fault && __atomic_load_n(s.h_fault, __ATOMIC_ACQUIRE) != 0u) {
+    *s.h_fault = 0u;
+    throw std::runtime_error(std::string(what) +
+                             ": packet traversal entered by a partial wave; the results were discarded");
+  }
 }
 
 template <typename T>
@@ -351,6 +371,7 @@ void build_scene_bvh() {
     throw std::runtime_error(err);
   static uint64_t generations = 0;
   s.scene.generation = ++generations;
+  s.scene.fault = s.d_fault;
   s.scene.sky_mode = s.host.sky_mode;
   s.scene.texels = s.d_texels;
   s.scene.textures = s.d_textures;
@@ -411,15 +432,15 @@ double render_compound(EyeState& e, const Cam& c) {
   } else if (e.shader == "single_dimension")
     ok = launch_frame_single_dim(e.d_colours, L.omm_count, s.d_frame, s.width, s.height, s.stream, &err);
   else if (e.shader == "raw_ommatidial_samples")
-    ok = e.counts.empty() ? launch_frame_raw_samples(e.d_samples, L.omm_count, e.spp, s.d_frame, s.width, s.height,
-                                                     s.stream, &err)
-                          : true;
+    ok = launch_frame_raw_samples(e.d_samples, e.counts.empty() ? nullptr : e.d_offsets, L.omm_count, e.spp,
+                                  s.d_frame, s.width, s.height, s.stream, &err);
   else
     ok = launch_frame_single_dim_fast(e.d_colours, L.omm_count, s.d_frame, s.width, s.stream, &err);
   if (!ok) throw std::runtime_error(err);
   hip_check(hipEventRecord(s.ev[3], s.stream), "event");
   hip_check(hipStreamSynchronize(s.stream), "render");
   const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  check_packet_fault("renderFrame");
   float f[3] = {0, 0, 0};
   for (int k = 0; k < 3; ++k) (void)hipEventElapsedTime(&f[k], s.ev[k], s.ev[k + 1]);
   s.timings[0] = f[0];
@@ -1048,6 +1069,34 @@ size_t getCurrentEyeSamples(float* samples, size_t count) {
   ABI_GUARD_END(return 0)
 }
 
+int probePacketGuard(const float* rays64, int activeLanes, int32_t* ids64) {
+  ABI_GUARD_BEGIN
+  State& s = st();
+  if (!s.loaded) throw std::runtime_error("probePacketGuard: no scene loaded");
+  if (!rays64 || !ids64 || activeLanes < 1 || activeLanes > 64)
+    throw std::runtime_error("probePacketGuard: 64 rays, 64 ids and 1..64 active lanes expected");
+  Ray* d_rays = nullptr;
+  int32_t* d_ids = nullptr;
+  std::string err;
+  bool ok = false;
+  hipError_t he = hipMalloc(&d_rays, sizeof(Ray) * 64);
+  if (he == hipSuccess) he = hipMalloc(&d_ids, sizeof(int32_t) * 64);
+  if (he == hipSuccess) he = hipMemcpy(d_rays, rays64, sizeof(Ray) * 64, hipMemcpyHostToDevice);
+  if (he == hipSuccess) {
+    ok = launch_packet_guard_probe(s.scene, d_rays, activeLanes, d_ids, s.stream, &err);
+    he = hipStreamSynchronize(s.stream);
+  }
+  if (ok && he == hipSuccess) he = hipMemcpy(ids64, d_ids, sizeof(int32_t) * 64, hipMemcpyDeviceToHost);
+  (void)hipFree(d_rays);
+  (void)hipFree(d_ids);
+  if (!ok && !err.empty()) throw std::runtime_error(err);
+  hip_check(he, "probePacketGuard");
+  const bool flagged = __atomic_load_n(s.h_fault, __ATOMIC_ACQUIRE) != 0u;
+  *s.h_fault = 0u;
+  return flagged ? 1 : 0;
+  ABI_GUARD_END(return -1)
+}
+
 int traceRays(const float* rays, size_t count, int32_t* primIds, float* tbb) {
   ABI_GUARD_BEGIN
   State& s = st();
@@ -1152,6 +1201,7 @@ double renderPoseBatch(const float* poses, size_t poseCount, float* rgbOut) {
   std::string err;
   double ms = -1.0;
   if (!s.batch.render(s.scene, U, ps, rgbOut, s.stream, &ms, &err)) throw std::runtime_error(err);
+  check_packet_fault("renderPoseBatch");
   return ms;
   ABI_GUARD_END(return -1.0)
 }
@@ -1174,6 +1224,21 @@ int setAdaptiveLabelError(float labelError) {
     return -1;
   }
   st().label_error = labelError;
+  return 0;
+}
+
+int getAdaptiveTimings(double* out4) {
+  if (!out4) return -1;
+  for (int k = 0; k < 4; ++k) out4[k] = st().adaptive_ms[k];
+  return 0;
+}
+
+int setAdaptiveSampler(int mode) {
+  if (mode != kSamplerNetwork && mode != kSamplerPooledSigma) {
+    set_error("setAdaptiveSampler: mode must be 0 (network) or 1 (pooled pilot deviation)");
+    return -1;
+  }
+  st().sampler = mode;
   return 0;
 }
 
@@ -1200,6 +1265,7 @@ int autotuneRenderBatch(const float* poses, size_t poseCount, int repeats, float
   UnitLaunch U = unit_launch_of(*e, nullptr, (int)poseCount);
   std::vector<TuneTrial> tr;
   const LaunchShape best = tune_shape(*e, U, ps, std::max(repeats, 1), &tr);
+  check_packet_fault("autotuneRenderBatch");
   // the winner becomes the shape of every later unit launch
   s.units_per_grab = best.units_per_grab;
   s.unit_blocks = best.blocks;
@@ -1333,6 +1399,7 @@ int64_t adaptiveTrainingSet(const float* poses, size_t poseCount, const uint8_t*
   if (!s.adaptive.training_set(s.scene, U, poses_from6(poses, poseCount), mask, pilot, labelSpp, errorTarget,
                                neighbours_of(*e), &f, &l, s.stream, &err))
     throw std::runtime_error(err);
+  check_packet_fault("adaptiveTrainingSet");
   s.label_error = errorTarget;
   const size_t rows = std::min(l.size(), maxRows);
   if (features) std::copy(f.begin(), f.begin() + (ptrdiff_t)(rows * kMlpIn), features);
@@ -1355,12 +1422,15 @@ double renderPoseBatchAdaptive(const float* poses, size_t poseCount, float error
   cfg.max_spp = maxSpp;
   cfg.budget_scale = budgetScale;
   cfg.neighbours = neighbours_of(*e);
+  cfg.sampler = s.sampler;
   AdaptiveResult res;
   std::string err;
   UnitLaunch U = unit_launch_of(*e, nullptr, (int)poseCount);
   if (!s.adaptive.render(s.scene, U, poses_from6(poses, poseCount), s.mlp, cfg, rgbOut, countsOut, &res, s.stream,
                          &err))
     throw std::runtime_error(err);
+  check_packet_fault("renderPoseBatchAdaptive");
+  for (int k = 0; k < 4; ++k) s.adaptive_ms[k] = res.stage_ms[k];
   if (totalSamples) *totalSamples = res.total_samples;
   return res.gpu_ms;
   ABI_GUARD_END(return -1.0)

@@ -114,8 +114,10 @@ bool launch_frame_single_dim_fast(const float4* colours, int omm_count, uchar4* 
 bool launch_frame_single_dim(const float4* colours, int omm_count, uchar4* frame, int width, int height,
                              hipStream_t stream, std::string* err);
 // raw_ommatidial_samples (shaders.cu:354-369): pixel (x = omm, y = sample).
-bool launch_frame_raw_samples(const float4* samples, int omm_count, int spp, uchar4* frame, int width,
-                              int height, hipStream_t stream, std::string* err);
+// offsets: prefix sum of per-ommatidium sample counts (omm_count + 1 entries),
+// or null for spp samples each; ommatidium x fills rows 0 .. count_x - 1.
+bool launch_frame_raw_samples(const float4* samples, const uint32_t* offsets, int omm_count, int spp,
+                              uchar4* frame, int width, int height, hipStream_t stream, std::string* err);
 
 // Frame of a regular camera (pinhole, orthographic or panoramic,
 // shaders.cu:198-333): every pixel traces its primary ray and shows
@@ -128,6 +130,12 @@ bool launch_generate_rays(const EyeLaunch& L, Ray* rays, hipStream_t stream, std
 
 // Closest hits of arbitrary rays: hit[i] = original triangle id or -1,
 // tbb[i] = (t, b1, b2, 0).
+// One wave traces rays[0..63] with the packet traversal; lanes at or above
+// active_lanes skip the call, so below 64 the traversal's partial-wave guard
+// sets the scene's fault word.  ids[lane]: hit id, -1 miss, -2 skipped.
+bool launch_packet_guard_probe(const DeviceScene& scene, const Ray* rays, int active_lanes, int32_t* ids,
+                               hipStream_t stream, std::string* err);
+
 bool launch_trace_rays(const DeviceScene& scene, const Ray* rays, uint64_t n, int32_t* hit,
                        float4* tbb, hipStream_t stream, std::string* err);
 

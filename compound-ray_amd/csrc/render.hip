@@ -26,6 +26,7 @@ struct SceneView {
   const TexView* textures;
   int node_count;
   int sky_mode;
+  uint32_t* fault;   // host-visible word set when a packet traversal is entered by a partial wave
 };
 
 SceneView view_of(const DeviceScene& s) {
@@ -37,6 +38,7 @@ SceneView view_of(const DeviceScene& s) {
   v.textures = s.textures;
   v.node_count = s.node_count;
   v.sky_mode = s.sky_mode;
+  v.fault = s.fault;
   return v;
 }
 
@@ -376,6 +378,14 @@ __device__ __forceinline__ void closest_hit_packet(const SceneView& S, const Ray
   h.k = 0xffffffffu;
   h.id = 0xffffffffu;
   if (S.node_count == 0) return;
+  // The stack's entry k lives in lane k & 63, written by a lane select: a
+  // lane outside EXEC would drop its entries.  Every caller enters with the
+  // whole wave; a partial wave is flagged to the host, which discards the
+  // launch's results, and returns misses instead of walking the tree.
+  if (__ballot(1) != ~0ull) {
+    if (S.fault) *S.fault = 1u;
+    return;
+  }
   if (__ballot(active && rp.par != 0) != 0ull)
     closest_hit_packet_impl<true, kScalar>(S, rp, active, h);
   else
@@ -804,6 +814,21 @@ __global__ __launch_bounds__(kTraceThreads, 6) void k_render_tasks(SceneView S, 
   if (lane == 0) U.colours[out] = make_float4(r, gg, bb, (float)n);
 }
 
+// Failure-handling probe of the packet traversal's partial-wave guard: one
+// wave traces 64 rays, lanes at or above active_lanes skip the call (their
+// ids stay 0xfffffffe).
+__global__ __launch_bounds__(64) void k_packet_guard_probe(SceneView S, const Ray* __restrict__ rays,
+                                                           int active_lanes, int32_t* __restrict__ ids) {
+  const int lane = (int)(threadIdx.x & 63);
+  Hit h;
+  h.id = 0xfffffffeu;
+  if (lane < active_lanes) {
+    const RayPre rp = precompute_ray(rays[lane]);
+    closest_hit_packet<false>(S, rp, true, h);
+  }
+  ids[lane] = (int32_t)h.id;
+}
+
 __global__ void k_fill_u32(uint32_t* __restrict__ p, uint32_t n, uint32_t v) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
@@ -898,14 +923,20 @@ __global__ void k_frame_sd(const float4* __restrict__ colours, int omm_count, uc
   frame[(size_t)y * width + x] = make_color(colours[i]);
 }
 
-__global__ void k_frame_raw(const float4* __restrict__ samples, int omm_count, int spp,
-                            uchar4* __restrict__ frame, int width, int height) {
+// Column x shows ommatidium x's samples, row y its sample y.  offsets
+// (nullable: spp samples each) is the per-ommatidium sample counts' prefix
+// sum; rows at or beyond an ommatidium's count keep the frame's contents.
+__global__ void k_frame_raw(const float4* __restrict__ samples, const uint32_t* __restrict__ offsets,
+                            int omm_count, int spp, uchar4* __restrict__ frame, int width, int height) {
   const int x = blockIdx.x * blockDim.x + threadIdx.x;
   const int y = blockIdx.y;
-  if (x >= width || y >= height || x >= omm_count || y >= spp) return;
+  if (x >= width || y >= height || x >= omm_count) return;
+  const size_t first = offsets ? (size_t)offsets[x] : (size_t)x * (size_t)spp;
+  const uint32_t n = offsets ? offsets[x + 1] - offsets[x] : (uint32_t)spp;
+  if ((uint32_t)y >= n) return;
   // the reference stores c / spp per sample and displays it unsummed
-  const float4 v = samples[(size_t)x * spp + y];
-  const float inv = 1.0f / (float)spp;
+  const float4 v = samples[first + (size_t)y];
+  const float inv = 1.0f / (float)n;
   frame[(size_t)y * width + x] = make_color(make_float4(v.x * inv, v.y * inv, v.z * inv, 0.0f));
 }
 
@@ -1154,11 +1185,11 @@ bool launch_frame_single_dim(const float4* colours, int omm_count, uchar4* frame
   return check_launch("frame", err);
 }
 
-bool launch_frame_raw_samples(const float4* samples, int omm_count, int spp, uchar4* frame, int width,
-                              int height, hipStream_t stream, std::string* err) {
+bool launch_frame_raw_samples(const float4* samples, const uint32_t* offsets, int omm_count, int spp,
+                              uchar4* frame, int width, int height, hipStream_t stream, std::string* err) {
   if (width <= 0 || height <= 0 || omm_count <= 0) return true;
-  hipLaunchKernelGGL(k_frame_raw, dim3((width + 255) / 256, height), dim3(256), 0, stream, samples, omm_count, spp,
-                     frame, width, height);
+  hipLaunchKernelGGL(k_frame_raw, dim3((width + 255) / 256, height), dim3(256), 0, stream, samples, offsets,
+                     omm_count, spp, frame, width, height);
   return check_launch("frame", err);
 }
 
@@ -1185,6 +1216,16 @@ bool launch_generate_rays(const EyeLaunch& L, Ray* rays, hipStream_t stream, std
   if (total == 0) return true;
   hipLaunchKernelGGL(k_generate_rays, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, stream, L, total, rays);
   return check_launch("generate_rays", err);
+}
+
+bool launch_packet_guard_probe(const DeviceScene& scene, const Ray* rays, int active_lanes, int32_t* ids,
+                               hipStream_t stream, std::string* err) {
+  if (scene.stack_bound > kPacketStackMax) {
+    if (err) *err = "packet probe: BVH too deep for the packet traversal";
+    return false;
+  }
+  hipLaunchKernelGGL(k_packet_guard_probe, dim3(1), dim3(64), 0, stream, view_of(scene), rays, active_lanes, ids);
+  return check_launch("packet_guard_probe", err);
 }
 
 bool launch_trace_rays(const DeviceScene& scene, const Ray* rays, uint64_t n, int32_t* hit, float4* tbb,

@@ -127,6 +127,8 @@ def configure_extensions(lib):
     lib.getCurrentEyeSamples.restype = c_size_t
     lib.traceRays.argtypes = [f32p, c_size_t, i32p, f32p]
     lib.traceRays.restype = c_int
+    lib.probePacketGuard.argtypes = [f32p, c_int, i32p]
+    lib.probePacketGuard.restype = c_int
     lib.renderPoseBatch.argtypes = [f32p, c_size_t, c_void_p]
     lib.renderPoseBatch.restype = c_double
     lib.adaptiveTrainingSet.argtypes = [c_void_p, c_size_t, c_void_p, c_int, c_int, c_float, c_void_p, c_void_p,
@@ -141,6 +143,10 @@ def configure_extensions(lib):
     lib.getAdaptiveStages.restype = c_int64
     lib.setAdaptiveLabelError.argtypes = [c_float]
     lib.setAdaptiveLabelError.restype = c_int
+    lib.setAdaptiveSampler.argtypes = [c_int]
+    lib.setAdaptiveSampler.restype = c_int
+    lib.getAdaptiveTimings.argtypes = [ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")]
+    lib.getAdaptiveTimings.restype = c_int
     lib.setTraversalMode.argtypes = [c_int]
     lib.setCurrentEyeSampleBase.argtypes = [c_uint32]
     lib.setSampleRecording.argtypes = [c_bool]
@@ -403,6 +409,14 @@ def render_adaptive_np(lib, poses6, error_target=0.01, max_spp=512, budget_scale
     if ms < 0:
         raise RuntimeError(last_error(lib))
     return ms, rgb, counts, int(total[0])
+
+
+def adaptive_timings_np(lib) -> np.ndarray:
+    """(pilot, allocation, top-up, combination) device ms of the last adaptive render."""
+    out = np.zeros(4, np.float64)
+    if lib.getAdaptiveTimings(out) != 0:
+        raise RuntimeError(last_error(lib))
+    return out
 
 
 def adaptive_stages_np(lib, units: int):

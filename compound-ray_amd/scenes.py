@@ -585,10 +585,13 @@ def config2_pose(scene: Scene, seed: int = 2) -> np.ndarray:
 
 
 def config5_scene(seed: int = 5, grid: int = 2048, extent: float = 80.0, n_cyl: int = 20000,
-                  n_rocks: int = 200) -> Scene:
+                  n_rocks: int = 200, cyl_segments: int = 64) -> Scene:
+    """2048^2 heightfield (8,380,418 triangles) + 20,000 64-segment cylinders
+    (3,840,000) + 200 subdivision-5 icosphere rocks (4,096,000): 16,316,418
+    triangles."""
     rng = np.random.default_rng(seed)
     terrain, hf = heightfield(grid, extent, 3.0, 7, 0.04, rng, colour="earth")
-    veg = cylinders(n_cyl, hf, rng, extent)
+    veg = cylinders(n_cyl, hf, rng, extent, segments=cyl_segments)
     stones = rocks(n_rocks, hf, rng, extent)
     return Scene([terrain, veg, stones], heightfield=hf)
 

@@ -154,6 +154,15 @@ size_t getCurrentEyeSamples(float* samples, size_t count);
  * Returns 0 on success, -1 on error. */
 int traceRays(const float* rays, size_t count, int32_t* primIds, float* tbb);
 
+/* Failure-handling check of the wave-packet traversal (no reference
+ * counterpart).  Its stack needs every lane of the wave: a packet traversal
+ * entered by a partial wave flags the call, which then fails with an error,
+ * and returns misses.  This traces rays64 (64 rays, 8 floats each) with one
+ * wave whose lanes at or above activeLanes (1..64) skip the traversal.
+ * ids64 receives each lane's hit id (-1 miss, -2 skipped).  Returns 1 when
+ * the guard flagged the launch, 0 when not, -1 on error. */
+int probePacketGuard(const float* rays64, int activeLanes, int32_t* ids64);
+
 /* Batched rendering of the current eye at many poses in one launch sequence:
  * poses[6*k..] = (posX, posY, posZ, rotX, rotY, rotZ) with setCameraPose
  * semantics; rgbOut receives 3 floats per (pose, ommatidium), pose-major.
@@ -191,13 +200,6 @@ int64_t adaptiveTrainingSet(const float* poses, size_t poseCount, const uint8_t*
  * (FP64, in that coordinate order), ties to the lower index; slots beyond
  * count - 1 neighbours repeat i.  k in 1..16; returns 0, or -1. */
 int nearestOmmatidia(const float* ommatidia, size_t count, int k, uint32_t* out);
-/* Batched adaptive render: 8-sample pilot, MLP prediction, per-pose Neyman
- * allocation for RMS error errorTarget (times budgetScale), top-up, combine.
- * The network predicts the samples needed for the label error (the
- * errorTarget of the last adaptiveTrainingSet call, or setAdaptiveLabelError);
- * allocations scale by (label error / errorTarget)^2.
- * rgbOut / countsOut (nullable) per (pose, ommatidium); *totalSamples gets the
- * samples traced.  Returns GPU milliseconds, or -1. */
 /* The two stages of the last renderPoseBatchAdaptive, per (pose, ommatidium)
  * unit: pilot and top-up colours as (r, g, b, samples) and the top-up sample
  * counts (any pointer may be NULL).  Returns maxUnits, or -1 when the last
@@ -205,6 +207,23 @@ int nearestOmmatidia(const float* ommatidia, size_t count, int k, uint32_t* out)
 int64_t getAdaptiveStages(float* pilot, float* topup, uint32_t* extra, size_t maxUnits);
 /* Error the sampler network's labels were made for (default 0.01). */
 int setAdaptiveLabelError(float labelError);
+/* Where renderPoseBatchAdaptive gets each unit's required samples: 0 (the
+ * default) the trained network; 1 the network-free estimate from the pooled
+ * pilot standard deviation of the unit and its 6 nearest ommatidia, sigma =
+ * (own + 6 * neighbours' mean) / 7, required samples sigma^2 / labelError^2.
+ * Everything after that step is shared.  Returns 0, or -1 for another mode. */
+int setAdaptiveSampler(int mode);
+/* Device milliseconds of the last renderPoseBatchAdaptive's stages, from HIP
+ * events on the library's stream: out4 = (pilot render, features + required
+ * samples + allocation, top-up render, combination).  Returns 0, or -1. */
+int getAdaptiveTimings(double* out4);
+/* Batched adaptive render: 8-sample pilot, MLP prediction, per-pose Neyman
+ * allocation for RMS error errorTarget (times budgetScale), top-up, combine.
+ * The network predicts the samples needed for the label error (the
+ * errorTarget of the last adaptiveTrainingSet call, or setAdaptiveLabelError);
+ * allocations scale by (label error / errorTarget)^2.
+ * rgbOut / countsOut (nullable) per (pose, ommatidium); *totalSamples gets the
+ * samples traced.  Returns GPU milliseconds, or -1. */
 double renderPoseBatchAdaptive(const float* poses, size_t poseCount, float errorTarget, int maxSpp,
                                float budgetScale, float* rgbOut, uint32_t* countsOut, uint64_t* totalSamples);
 

@@ -422,3 +422,84 @@ def test_batch_graph_rebuilt_with_scene(lib, config1, oracle_c1, tmp_path):
         assert np.array_equal(rgb1, want0)
     finally:
         assert lib.setSceneLeafSize(4) == 0
+
+
+def test_packet_guard_flags_partial_waves(lib, config1, oracle_c1):
+    """The wave-packet traversal keeps its stack in the lanes of the wave, so
+    it needs every lane.  Entered by the whole wave it returns the oracle's
+    hits and flags nothing; entered by a partial wave (lanes skipping the
+    call, forced from the host) it flags the launch and returns misses
+    without walking the tree; the flag is cleared and later renders are
+    unaffected."""
+    setup_eye(lib, config1, spp=32)
+    rays = np.ascontiguousarray(er.rays_np(lib)[::500][:64])
+    assert rays.shape == (64, 8)
+    oids, _ = oracle_c1.trace(rays)
+    assert (oids >= 0).any() and (oids < 0).any()
+    ids = np.zeros(64, np.int32)
+    assert lib.probePacketGuard(rays, 64, ids) == 0, er.last_error(lib)
+    assert np.array_equal(ids, oids)
+    for active in (1, 32, 63):
+        ids[:] = 0
+        assert lib.probePacketGuard(rays, active, ids) == 1
+        assert np.all(ids[:active] == -1) and np.all(ids[active:] == -2)
+    assert lib.probePacketGuard(rays, 64, ids) == 0
+    assert np.array_equal(ids, oids)
+    assert lib.renderFrame() >= 0, er.last_error(lib)
+    want = oracle_c1.render(config1["eyes"][0], er.pose_np(lib), spp=32, frame=lib.getCurrentEyeFrameIndex() - 1)
+    assert np.array_equal(er.colours_np(lib), want["colour_gpu_order"][0])
+
+
+@pytest.mark.parametrize("width,height", [(333, 3), (1500, 2)])
+def test_single_dimension_projection(lib, config1, oracle_c1, width, height):
+    """single_dimension (shaders.cu:375-389): pixel (x, y) shows the summed
+    colour of ommatidium x * n / width on every row."""
+    omm = config1["eyes"][0]
+    setup_eye(lib, config1, omm=omm, spp=8, record=False)
+    lib.setCurrentEyeShaderName(b"single_dimension")
+    er.setRenderSize(lib, width, height)
+    try:
+        assert lib.renderFrame() >= 0, er.last_error(lib)
+        want = oracle_c1.render(omm, er.pose_np(lib), spp=8, frame=lib.getCurrentEyeFrameIndex() - 1)
+        col = want["colour_gpu_order"][0]
+        assert np.array_equal(er.colours_np(lib), col)
+        frame = np.array(lib.getFramePointer(), copy=True).reshape(height, width, 4)
+        idx = (np.arange(width, dtype=np.int64) * len(omm)) // width
+        assert np.array_equal(frame, np.broadcast_to(oracle.make_color(col)[idx], (height, width, 4)))
+    finally:
+        lib.setCurrentEyeShaderName(b"single_dimension_fast")
+        er.setRenderSize(lib, 400, 400)
+
+
+@pytest.mark.parametrize("uneven", [False, True])
+def test_raw_ommatidial_samples_projection(lib, config1, oracle_c1, uneven):
+    """raw_ommatidial_samples (shaders.cu:354-368): column x is ommatidium x,
+    row y its sample y pre-scaled by 1/n (n: that ommatidium's sample count,
+    spp when uniform), through make_color.  Rows at or beyond n, and
+    columns beyond the ommatidia, are not written."""
+    n_omm, spp = 150, 12
+    omm = config1["eyes"][0][:n_omm]
+    setup_eye(lib, config1, omm=omm, spp=spp, record=False)
+    counts = None
+    if uneven:
+        counts = np.random.default_rng(8).integers(0, 20, size=n_omm).astype(np.uint32)
+        counts[:3] = [0, 1, 19]
+        er.set_sample_counts(lib, counts)
+    lib.setCurrentEyeShaderName(b"raw_ommatidial_samples")
+    w, h = 160, 24
+    er.setRenderSize(lib, w, h)
+    try:
+        assert lib.renderFrame() >= 0, er.last_error(lib)
+        frame_index = lib.getCurrentEyeFrameIndex() - 1
+        ref = oracle_c1.render(omm, er.pose_np(lib), spp=spp, counts=counts, frame=frame_index, want_samples=True)
+        n = np.full(n_omm, spp, np.int64) if counts is None else counts.astype(np.int64)
+        first = np.concatenate([[0], np.cumsum(n)])
+        frame = np.array(lib.getFramePointer(), copy=True).reshape(h, w, 4)
+        for x in range(n_omm):
+            s = ref["samples"][first[x]:first[x + 1], :3]
+            inv = np.float32(1.0) / np.float32(n[x])
+            assert np.array_equal(frame[:n[x], x], oracle.make_color((s * inv).astype(np.float32))), x
+    finally:
+        lib.setCurrentEyeShaderName(b"single_dimension_fast")
+        er.set_sample_counts(lib, None)
+        er.setRenderSize(lib, 400, 400)
