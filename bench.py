@@ -516,12 +516,21 @@ def bench_large(lib, er, scenes, tmp: str, steps: int, warmup: int, target: floa
     st_top = {k: st_all[k] - st_pilot[k] for k in st_all}
     # pilot: ommatidium 32 B read, colour 16 B + statistics 32 B written per
     # unit; top-up: ommatidium 32 B + count 4 B read, colour 16 B written
-    b_pilot = traversal_bytes(st_pilot, n_omm, 32 + 16 + 32)
-    b_top = traversal_bytes(st_top, n_omm, 32 + 4 + 16)
+    unit_io = n_omm * ((32 + 16 + 32) + (32 + 4 + 16))
+    b_per_ray = traversal_bytes(st_pilot, 0, 0) + traversal_bytes(st_top, 0, 0) + unit_io
+    # what the waves fetch: a packet reads each node it visits and the
+    # triangles of each leaf it enters once for its 64 rays (counted on the
+    # GPU in an untimed run of the same render); shading gathers stay per ray
+    lib.setPacketCounting(True)
+    er.packet_counts(lib)
+    adaptive(scale, want_rgb=False)
+    pk = er.packet_counts(lib)
+    lib.setPacketCounting(False)
+    b_packet = pk["nodes"] * 128.0 + pk["tris"] * 48.0 + st_all["hits"] * 48.0 + unit_io
     ms_pilot = float(stage[:, 0].mean())
     ms_top = float(stage[:, 2].mean())
     launch_ms = (ms_pilot + ms_top) / 2
-    bytes_launch = (b_pilot + b_top) / 2
+    bytes_launch = b_packet / 2
     achieved = bytes_launch / (launch_ms * 1e-3) / 1e9
     traffic = measured_traffic("k_render_units", "c5")
     out = {
@@ -554,6 +563,11 @@ def bench_large(lib, er, scenes, tmp: str, steps: int, warmup: int, target: floa
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": "k_render_units",
                      "bytes_per_launch": bytes_launch, "launch_ms": launch_ms,
                      "launches": "pilot and top-up of one adaptive render (HIP events on the library stream)",
+                     "bytes": "per wave: packet node visits x 128 B + leaf triangles x 48 B; per hit 48 B of "
+                              "shading data; per unit 132 B of ommatidium, count, colour and statistics",
+                     "packet_nodes_per_64_rays": pk["nodes"] * 64.0 / total,
+                     "packet_tris_per_64_rays": pk["tris"] * 64.0 / total,
+                     "per_ray_equivalent_GBps": b_per_ray / 2 / (launch_ms * 1e-3) / 1e9,
                      "hbm_frac_measured": None if traffic is None else traffic / (launch_ms * 1e-3) / 1e9 /
                      HBM_PEAK_GBS},
         "launch_shape": shape,
@@ -601,6 +615,61 @@ def bench_large(lib, er, scenes, tmp: str, steps: int, warmup: int, target: floa
     return out
 
 
+def bench_speed_test(lib, er, tmp: str, frames: int = 200, spp_list=(1, 2, 4, 8, 16, 32, 64, 128, 256, 512,
+                                                                        1000, 2000, 3200)) -> dict:
+    """The reference's speed test (python-examples/speed-test/speedTest.py:
+    81-124) on the scene it ships (data/natural-standin-sky.gltf; its lab and
+    natural speed-test scenes are not shipped): the 1,000-ommatidium
+    equidistant eye through the first compound camera, single_dimension_fast
+    into a 1000 x 1 frame, a warm-up, then for each sample count two untimed
+    frames and the mean of renderFrame()'s host wall milliseconds over
+    `frames` frames (the reference averages 500)."""
+    import gzip
+    golden = ROOT / "tests" / "golden"
+    d = pathlib.Path(tmp) / "speedtest"
+    (d / "eyes").mkdir(parents=True, exist_ok=True)
+    gltf = d / "natural-standin-sky.gltf"
+    gltf.write_bytes(gzip.decompress((golden / "natural_standin_sky.gltf.gz").read_bytes()))
+    eye = gzip.decompress((golden / "standin_eye.eye.gz").read_bytes())
+    for where in (d, d / "eyes"):
+        (where / "1000-horizontallyAcute-variableDegree.eye").write_bytes(eye)
+    rows = [[float(t) for t in ln.split()] for ln in
+            gzip.decompress((golden / "equidistant_eye.eye.gz").read_bytes()).decode().splitlines() if ln.strip()]
+    lib.loadGlTFscene(str(gltf).encode())
+    if er.last_error(lib):
+        raise RuntimeError(er.last_error(lib))
+    er.gotoFirstCompoundEye(lib)
+    er.set_ommatidia_np(lib, np.asarray(rows, np.float32))
+    lib.setCurrentEyeShaderName(b"single_dimension_fast")
+    er.setRenderSize(lib, lib.getCurrentEyeOmmatidialCount(), 1)
+    lib.setCurrentEyeSamplesPerOmmatidium(1)
+    t0 = time.time()
+    while time.time() - t0 < 1.0:
+        lib.renderFrame()
+    out = {}
+    for spp in spp_list:
+        lib.setCurrentEyeSamplesPerOmmatidium(spp)
+        lib.renderFrame()
+        lib.renderFrame()
+        total = 0.0
+        for _ in range(frames):
+            ms = lib.renderFrame()
+            if ms < 0:
+                raise RuntimeError(er.last_error(lib))
+            total += ms
+        out[str(spp)] = total / frames
+    n = lib.getCurrentEyeOmmatidialCount()
+    return {
+        "workload": "reference speed test: natural-standin-sky.gltf (24,200 triangles, textured), 1,000-ommatidium "
+                    f"equidistant eye, single_dimension_fast 1000x1, mean renderFrame() host ms over {frames} frames",
+        "ms_per_frame": out,
+        "grays_per_s": {k: n * int(k) / v / 1e6 for k, v in out.items()},
+        "reference_rtx_2080_ti_natural_ms": {"32": 0.2096, "128": 0.2576, "1000": 0.6804, "3200": 1.7482},
+        "reference_gtx_1080_ti_natural_ms": {"1000": 4.3611, "3200": 13.5052},
+        "note": "the reference's figures are on its own natural scene (not shipped); context, not a target",
+    }
+
+
 def bench_plain(lib, er, cfg: dict, eyes: list, steps: int, warmup: int, trace_mode: int, copy_probe: float,
                 cpu: bool, rms: bool) -> dict:
     """The plain method at bee scale: both eyes through renderFrame at 128
@@ -625,6 +694,11 @@ def bench_plain(lib, er, cfg: dict, eyes: list, steps: int, warmup: int, trace_m
             if lib.renderFrame() < 0:
                 raise RuntimeError(er.last_error(lib))
 
+    lib.setPacketCounting(True)
+    er.packet_counts(lib)
+    step()
+    pk = er.packet_counts(lib)
+    lib.setPacketCounting(False)
     for _ in range(warmup):
         step()
     trace_ms = []
@@ -643,6 +717,8 @@ def bench_plain(lib, er, cfg: dict, eyes: list, steps: int, warmup: int, trace_m
     # rays' own traversals count them (a packet fetches a node once per wave)
     bytes_launch = traversal_bytes(stats, omm_total, 32 + 16) / len(eyes)
     achieved = bytes_launch / (trace_avg_ms * 1e-3) / 1e9
+    # what the waves fetch (each node and leaf once per 64-ray packet)
+    packet_launch = (pk["nodes"] * 128.0 + pk["tris"] * 48.0 + stats["hits"] * 48.0 + omm_total * 48.0) / len(eyes)
     traffic = measured_traffic(kernel_name, "c2")
     if copy_probe > 0:
         nbytes = int(copy_probe * 2 ** 20)
@@ -660,8 +736,11 @@ def bench_plain(lib, er, cfg: dict, eyes: list, steps: int, warmup: int, trace_m
                               "tris": stats["tris_tested"] / rays_per_step,
                               "hit_fraction": stats["hits"] / rays_per_step},
         "roofline": {"bound": "latency of dependent node fetches (L2/MALL-resident scene)",
-                     "achieved_algorithmic": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac_algorithmic": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "achieved_packet": packet_launch / (trace_avg_ms * 1e-3) / 1e9,
+                     "frac_packet": packet_launch / (trace_avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                     "per_ray_equivalent_GBps": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "packet_nodes_per_64_rays": pk["nodes"] * 64.0 / rays_per_step,
+                     "traffic": traffic,
                      "hbm_frac_measured": None if traffic is None else traffic / (trace_avg_ms * 1e-3) / 1e9 /
                      HBM_PEAK_GBS,
                      "kernel": kernel_name, "bytes_per_launch": bytes_launch, "trace_mode": trace_mode},
@@ -696,6 +775,7 @@ def main() -> int:
     ap.add_argument("--skip-batched", action="store_true", help="skip the 4,096-pose batched configuration")
     ap.add_argument("--skip-adaptive", action="store_true", help="skip learned adaptive sampling at bee scale")
     ap.add_argument("--skip-plain", action="store_true", help="skip the bee-scale renderFrame workload")
+    ap.add_argument("--skip-speed-test", action="store_true", help="skip the reference speed-test sweep")
     ap.add_argument("--skip-large", action="store_true",
                     help="skip the headline large batch (the line then carries no headline value)")
     ap.add_argument("--adaptive-only", action="store_true",
@@ -809,6 +889,9 @@ def main() -> int:
     if not only and not args.skip_batched:
         log("batched configuration ...")
         out["batched"] = bench_batched(lib, er, scenes, tmp)
+    if not only and not args.skip_speed_test:
+        log("reference speed-test sweep ...")
+        out["speed_test"] = bench_speed_test(lib, er, tmp)
     lib.stop()
     print(json.dumps(out))
     return 0

@@ -91,6 +91,9 @@ class AdaptiveRenderer {
   uint32_t* d_counts_ = nullptr;  // final sample count per unit
   uint32_t* d_extra_ = nullptr;   // top-up samples per unit
   unsigned long long* d_total_ = nullptr;
+  float* d_part_ = nullptr;       // partial sums of the allocation
+  size_t part_cap_ = 0;
+  float* d_sqrt_ = nullptr;       // per pose: sum of sqrt(n_req)
   uint32_t* d_queue_ = nullptr;
   size_t units_cap_ = 0;
   hipEvent_t ev0_ = nullptr, ev1_ = nullptr;

@@ -72,37 +72,77 @@ __device__ inline float required_samples(float log2_nreq) {
   return exp2f(fminf(fmaxf(log2_nreq, 0.0f), 20.0f));
 }
 
-// One block per pose: S = sum sqrt(n_req) over the pose's ommatidia (fixed
-// order: per-thread strided sums, then a tree in LDS), then the counts.
-__global__ __launch_bounds__(kThreads) void k_allocate(const float* __restrict__ pred, int omm_count, int pilot,
-                                                      int max_spp, float scale, uint32_t* __restrict__ counts,
-                                                      uint32_t* __restrict__ extra,
-                                                      unsigned long long* __restrict__ total) {
-  __shared__ float part[kThreads];
-  const uint64_t base = (uint64_t)blockIdx.x * (uint64_t)omm_count;
-  float s = 0.0f;
-  for (int i = threadIdx.x; i < omm_count; i += kThreads) s += sqrtf(required_samples(pred[base + i]));
-  part[threadIdx.x] = s;
+// Allocation in three steps, every step spread over the whole GPU (a batch
+// can hold one pose of millions of units) and summed in a fixed order, so the
+// counts do not depend on scheduling:
+//   1. k_alloc_partial: block (pose, c) sums sqrt(n_req) over units
+//      [c * kAllocChunk, (c + 1) * kAllocChunk) of the pose;
+//   2. k_alloc_total: S[pose] = the sum of the pose's partial sums;
+//   3. k_allocate: every unit's count from S[pose], and the total.
+constexpr int kAllocChunk = 4096;
+
+__device__ inline float block_sum(float v, float* part) {
+  part[threadIdx.x] = v;
   __syncthreads();
   for (int w = kThreads / 2; w >= 1; w >>= 1) {
     if (threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
     __syncthreads();
   }
-  const float S = part[0];
+  return part[0];
+}
+
+__global__ __launch_bounds__(kThreads) void k_alloc_partial(const float* __restrict__ pred, int omm_count, int chunks,
+                                                           float* __restrict__ partial) {
+  __shared__ float part[kThreads];
+  const int pose = (int)(blockIdx.x / (unsigned)chunks);
+  const int c = (int)(blockIdx.x - (unsigned)pose * (unsigned)chunks);
+  const uint64_t base = (uint64_t)pose * (uint64_t)omm_count;
+  const int i1 = min(omm_count, (c + 1) * kAllocChunk);
+  float s = 0.0f;
+  for (int i = c * kAllocChunk + (int)threadIdx.x; i < i1; i += kThreads) s += sqrtf(required_samples(pred[base + i]));
+  const float t = block_sum(s, part);
+  if (threadIdx.x == 0) partial[blockIdx.x] = t;
+}
+
+__global__ __launch_bounds__(kThreads) void k_alloc_total(const float* __restrict__ partial, int chunks,
+                                                         float* __restrict__ total_sqrt) {
+  __shared__ float part[kThreads];
+  float s = 0.0f;
+  for (int c = (int)threadIdx.x; c < chunks; c += kThreads) s += partial[(size_t)blockIdx.x * chunks + c];
+  const float t = block_sum(s, part);
+  if (threadIdx.x == 0) total_sqrt[blockIdx.x] = t;
+}
+
+// n_i = sqrt(n_req,i) * S / N * scale, clamped to [pilot, max_spp].
+__global__ __launch_bounds__(kThreads) void k_allocate(const float* __restrict__ pred, int omm_count, uint64_t units,
+                                                      const float* __restrict__ total_sqrt, int pilot, int max_spp,
+                                                      float scale, uint32_t* __restrict__ counts,
+                                                      uint32_t* __restrict__ extra,
+                                                      unsigned long long* __restrict__ total) {
+  __shared__ unsigned long long part[kThreads];
+  const uint64_t u = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
   unsigned long long mine = 0;
-  for (int i = threadIdx.x; i < omm_count; i += kThreads) {
-    const float want = sqrtf(required_samples(pred[base + i])) * S / (float)omm_count * scale;
+  if (u < units) {
+    const uint64_t pose = u / (uint64_t)omm_count;
+    const float S = total_sqrt[pose];
+    const float want = sqrtf(required_samples(pred[u])) * S / (float)omm_count * scale;
     const float c = fminf(fmaxf(ceilf(want), (float)pilot), (float)max_spp);
     // The top-up pass gives a unit of e samples a power-of-two lane segment
     // (e <= 32) or whole 64-lane rounds (e > 32); those lanes cost the same
     // time idle or busy, so the top-up is rounded up to fill them.
     const uint32_t e = lane_footprint((uint32_t)c - (uint32_t)pilot);
     const uint32_t n = min((uint32_t)pilot + e, (uint32_t)max_spp);
-    counts[base + i] = n;
-    extra[base + i] = n - (uint32_t)pilot;
-    mine += n;
+    counts[u] = n;
+    extra[u] = n - (uint32_t)pilot;
+    mine = n;
   }
-  atomicAdd(total, mine);
+  part[threadIdx.x] = mine;
+  __syncthreads();
+  for (int w = kThreads / 2; w >= 1; w >>= 1) {
+    if (threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) atomicAdd(total, part[0]);
 }
 
 // Final colour of each unit into `out`: the pilot mean alone, or pilot and
@@ -174,9 +214,12 @@ bool ok_launch(const char* what, std::string* err) {
 
 void AdaptiveRenderer::release() {
   for (void* p : {(void*)d_poses_, (void*)d_col_a_, (void*)d_col_b_, (void*)d_final_, (void*)d_stats_, (void*)d_feat_,
-                  (void*)d_pred_, (void*)d_counts_, (void*)d_extra_, (void*)d_total_, (void*)d_queue_})
+                  (void*)d_pred_, (void*)d_counts_, (void*)d_extra_, (void*)d_total_, (void*)d_queue_,
+                  (void*)d_part_, (void*)d_sqrt_})
     if (p) (void)hipFree(p);
   d_poses_ = nullptr;
+  d_part_ = d_sqrt_ = nullptr;
+  part_cap_ = 0;
   d_col_a_ = d_col_b_ = d_final_ = d_stats_ = nullptr;
   d_feat_ = d_pred_ = nullptr;
   d_counts_ = d_extra_ = d_queue_ = nullptr;
@@ -200,8 +243,14 @@ bool AdaptiveRenderer::ensure(size_t poses, size_t units, std::string* err) {
   }
   if (!d_total_ && (!grow(&d_total_, 1, err) || !grow(&d_queue_, kQueueWords, err))) return false;
   if (poses > poses_cap_) {
-    if (!grow(&d_poses_, poses, err)) return false;
+    if (!grow(&d_poses_, poses, err) || !grow(&d_sqrt_, poses, err)) return false;
     poses_cap_ = poses;
+  }
+  // per-pose partial sums of the allocation: ceil(units per pose / chunk) each
+  const size_t parts = units == 0 ? 0 : poses * ((units / poses + kAllocChunk - 1) / kAllocChunk);
+  if (parts > part_cap_) {
+    if (!grow(&d_part_, parts, err)) return false;
+    part_cap_ = parts;
   }
   if (units > units_cap_) {
     if (!grow(&d_col_a_, units, err) || !grow(&d_col_b_, units, err) || !grow(&d_final_, units, err) ||
@@ -271,9 +320,13 @@ bool AdaptiveRenderer::render(const DeviceScene& scene, UnitLaunch U, const std:
     if (!ok_launch("pooled sigma", err)) return false;
   }
   hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, stream, d_total_);
-  hipLaunchKernelGGL(k_allocate, dim3((unsigned)np), dim3(kThreads), 0, stream, d_pred_, U.omm_count, cfg.pilot,
-                     cfg.max_spp, cfg.budget_scale * (cfg.label_error / cfg.error_target) *
-                                      (cfg.label_error / cfg.error_target),
+  const int chunks = (U.omm_count + kAllocChunk - 1) / kAllocChunk;
+  hipLaunchKernelGGL(k_alloc_partial, dim3((unsigned)(np * (size_t)chunks)), dim3(kThreads), 0, stream, d_pred_,
+                     U.omm_count, chunks, d_part_);
+  hipLaunchKernelGGL(k_alloc_total, dim3((unsigned)np), dim3(kThreads), 0, stream, d_part_, chunks, d_sqrt_);
+  hipLaunchKernelGGL(k_allocate, dim3(gb), dim3(kThreads), 0, stream, d_pred_, U.omm_count, (uint64_t)units, d_sqrt_,
+                     cfg.pilot, cfg.max_spp,
+                     cfg.budget_scale * (cfg.label_error / cfg.error_target) * (cfg.label_error / cfg.error_target),
                      d_counts_, d_extra_, d_total_);
   if (!ok_launch("allocate", err)) return false;
   (void)hipEventRecord(ev_stage_[1], stream);

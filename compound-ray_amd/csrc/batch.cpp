@@ -31,7 +31,8 @@ bool same_launch(const UnitLaunch& a, const UnitLaunch& b) {
          a.omm_counts == b.omm_counts && a.unit_counts == b.unit_counts && a.sample_base == b.sample_base &&
          a.order == b.order && a.colours == b.colours && a.stats == b.stats && a.samples == b.samples &&
          a.sample_offsets == b.sample_offsets && a.queue == b.queue && a.units_per_grab == b.units_per_grab &&
-         a.auto_grab_max == b.auto_grab_max && a.blocks == b.blocks && a.traversal == b.traversal;
+         a.auto_grab_max == b.auto_grab_max && a.blocks == b.blocks && a.traversal == b.traversal &&
+         a.packet_counts == b.packet_counts;
 }
 }  // namespace
 

@@ -122,6 +122,8 @@ struct State {
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   double timings[4] = {0, 0, 0, 0};   // trace, reduce, projection, total GPU ms of the last render
   unsigned long long* d_stats = nullptr;
+  unsigned long long* d_packet_counts = nullptr;  // packet node visits, leaf triangles (when counting)
+  bool count_packets = false;
   uint32_t* h_fault = nullptr;   // mapped host word: packet traversal entered by a partial wave
   uint32_t* d_fault = nullptr;   // its device address
   BatchRenderer batch;
@@ -170,6 +172,8 @@ void ensure_hip() {
   hip_check(hipMalloc(&s.d_pose, sizeof(Pose)), "hipMalloc pose");
   hip_check(hipMalloc(&s.d_stats, sizeof(unsigned long long) * 4), "hipMalloc stats");
   hip_check(hipMalloc(&s.d_queue, sizeof(uint32_t) * kQueueWords), "hipMalloc queue");
+  hip_check(hipMalloc(&s.d_packet_counts, sizeof(unsigned long long) * 2), "hipMalloc packet counts");
+  hip_check(hipMemset(s.d_packet_counts, 0, sizeof(unsigned long long) * 2), "clear packet counts");
   for (hipEvent_t& e : s.ev) hip_check(hipEventCreate(&e), "hipEventCreate");
   hip_check(hipHostMalloc(&s.h_fault, sizeof(uint32_t), hipHostMallocMapped | hipHostMallocCoherent),
             "hipHostMalloc fault word");
@@ -350,6 +354,7 @@ UnitLaunch unit_launch_of(EyeState& e, const Pose* d_poses, int pose_count) {
   U.blocks = s.unit_blocks;
   U.traversal = unit_traversal_of(trace_mode());
   U.order = e.d_order;
+  U.packet_counts = s.count_packets ? s.d_packet_counts : nullptr;
   return U;
 }
 
@@ -1225,6 +1230,23 @@ int setAdaptiveLabelError(float labelError) {
   }
   st().label_error = labelError;
   return 0;
+}
+
+void setPacketCounting(bool enabled) { st().count_packets = enabled; }
+
+int getPacketCounts(uint64_t* out2) {
+  ABI_GUARD_BEGIN
+  State& s = st();
+  if (!out2) throw std::runtime_error("getPacketCounts: null output");
+  ensure_hip();
+  unsigned long long h[2] = {0, 0};
+  hip_check(hipStreamSynchronize(s.stream), "sync");
+  hip_check(hipMemcpy(h, s.d_packet_counts, sizeof(h), hipMemcpyDeviceToHost), "packet counts readback");
+  hip_check(hipMemset(s.d_packet_counts, 0, sizeof(h)), "clear packet counts");
+  out2[0] = h[0];
+  out2[1] = h[1];
+  return 0;
+  ABI_GUARD_END(return -1)
 }
 
 int getAdaptiveTimings(double* out4) {

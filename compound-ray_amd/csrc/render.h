@@ -62,6 +62,9 @@ struct UnitLaunch {
   int auto_grab_max = 16;
   int blocks = 0;                       // blocks of 256 threads; 0: every resident slot once
   int traversal = 0;                    // UnitTraversal
+  // nullable: packet traversals add their node visits ([0]) and the triangles
+  // of the leaves they enter ([1]), counted once per wave (measurement runs)
+  unsigned long long* packet_counts = nullptr;
 };
 
 // Traversal inside the unit renderer: the wave walks the BVH as one packet

@@ -291,7 +291,7 @@ __device__ __forceinline__ int stack_top(int s0, int s1, int sp) {
 // ic / ik (child, ordering key).
 template <bool kParallel, bool kScalar>
 __device__ __forceinline__ void packet_visit(const SceneView& S, const RayPre& rp, bool active, int node, Hit& h,
-                                             int* ic, uint32_t* ik, int& ni) {
+                                             int* ic, uint32_t* ik, int& ni, uint32_t& tris) {
   float4 lx, hx, ly, hy, lz, hz;
   int4 ch;
   if (kScalar) {
@@ -317,6 +317,7 @@ __device__ __forceinline__ void packet_visit(const SceneView& S, const RayPre& r
     const bool mine = active && te[s] < INFINITY;
     if (c == kChildEmpty || __ballot(mine) == 0ull) continue;
     if (child_is_leaf(c)) {
+      tris += leaf_count(c);
       // inactive lanes test a dummy ray whose result is discarded
       if (kScalar)
         test_leaf(UniformPtr{S.tris}, c, rp, h);
@@ -333,8 +334,11 @@ __device__ __forceinline__ void packet_visit(const SceneView& S, const RayPre& r
   }
 }
 
+// steps / tris count the packet's node visits and the triangles of the leaves
+// it enters (wave-uniform: what the wave fetches from memory).
 template <bool kParallel, bool kScalar>
-__device__ __forceinline__ void closest_hit_packet_impl(const SceneView& S, const RayPre& rp, bool active, Hit& h) {
+__device__ __forceinline__ void closest_hit_packet_impl(const SceneView& S, const RayPre& rp, bool active, Hit& h,
+                                                        uint32_t& steps, uint32_t& tris) {
   int sp = 0;
   int node = 0;
   int stk0 = 0, stk1 = 0;
@@ -344,7 +348,8 @@ __device__ __forceinline__ void closest_hit_packet_impl(const SceneView& S, cons
     int ni = 0;
     int ic[4];
     uint32_t ik[4];
-    packet_visit<kParallel, kScalar>(S, rp, active, node, h, ic, ik, ni);
+    ++steps;
+    packet_visit<kParallel, kScalar>(S, rp, active, node, h, ic, ik, ni, tris);
     if (ni == 0) {
       if (sp == 0) break;
       --sp;
@@ -372,7 +377,8 @@ __device__ __forceinline__ void closest_hit_packet_impl(const SceneView& S, cons
 }
 
 template <bool kScalar = false>
-__device__ __forceinline__ void closest_hit_packet(const SceneView& S, const RayPre& rp, bool active, Hit& h) {
+__device__ __forceinline__ void closest_hit_packet(const SceneView& S, const RayPre& rp, bool active, Hit& h,
+                                                   uint32_t& steps, uint32_t& tris) {
   h.t = rp.tmax;
   h.b1 = h.b2 = 0.0f;
   h.k = 0xffffffffu;
@@ -387,9 +393,9 @@ __device__ __forceinline__ void closest_hit_packet(const SceneView& S, const Ray
     return;
   }
   if (__ballot(active && rp.par != 0) != 0ull)
-    closest_hit_packet_impl<true, kScalar>(S, rp, active, h);
+    closest_hit_packet_impl<true, kScalar>(S, rp, active, h, steps, tris);
   else
-    closest_hit_packet_impl<false, kScalar>(S, rp, active, h);
+    closest_hit_packet_impl<false, kScalar>(S, rp, active, h, steps, tris);
 }
 
 __device__ __forceinline__ float wave_sum(float v) {
@@ -471,6 +477,7 @@ __global__ __launch_bounds__(kTraceThreads, kTrav == kUnitPerRay ? 4 : 6) void k
   const uint64_t slice = (units + kXcds - 1) / kXcds;
   uint32_t x = blockIdx.x % kXcds;
   uint32_t tried = 0;
+  uint32_t w_steps = 0, w_tris = 0;   // packet node visits and leaf triangles of this wave
   for (;;) {
     uint64_t base = 0, s_end = 0;
     while (tried < kXcds) {
@@ -486,7 +493,13 @@ __global__ __launch_bounds__(kTraceThreads, kTrav == kUnitPerRay ? 4 : 6) void k
       x = (x + 1u) % kXcds;
       ++tried;
     }
-    if (tried >= kXcds) break;
+    if (tried >= kXcds) {
+      if (U.packet_counts && lane == 0 && w_steps != 0u) {
+        atomicAdd(U.packet_counts, (unsigned long long)w_steps);
+        atomicAdd(U.packet_counts + 1, (unsigned long long)w_tris);
+      }
+      break;
+    }
     const int G = (int)min((uint64_t)U.units_per_grab, s_end - base);
     // lane g < G holds the sample count of unit base + g
     uint32_t my_n = 0;
@@ -564,7 +577,7 @@ __global__ __launch_bounds__(kTraceThreads, kTrav == kUnitPerRay ? 4 : 6) void k
           h.b1 = h.b2 = 0.0f;
           if (active) closest_hit(S, rp, lane_stack, (int)blockDim.x, h);
         } else {
-          closest_hit_packet<kTrav == kUnitPacketScalar>(S, rp, active, h);
+          closest_hit_packet<kTrav == kUnitPacketScalar>(S, rp, active, h, w_steps, w_tris);
         }
         if (active) {
           const V3 c = shade(S, h, ray.dx, ray.dy, ray.dz);
@@ -785,7 +798,12 @@ __global__ __launch_bounds__(kTraceThreads, 6) void k_render_tasks(SceneView S, 
   }
   const RayPre rp = precompute_ray(ray);
   Hit h;
-  closest_hit_packet<kScalar>(S, rp, active, h);
+  uint32_t w_steps = 0, w_tris = 0;
+  closest_hit_packet<kScalar>(S, rp, active, h, w_steps, w_tris);
+  if (U.packet_counts && lane == 0) {
+    atomicAdd(U.packet_counts, (unsigned long long)w_steps);
+    atomicAdd(U.packet_counts + 1, (unsigned long long)w_tris);
+  }
   float4 a = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   if (active) {
     const V3 c = shade(S, h, ray.dx, ray.dy, ray.dz);
@@ -824,7 +842,8 @@ __global__ __launch_bounds__(64) void k_packet_guard_probe(SceneView S, const Ra
   h.id = 0xfffffffeu;
   if (lane < active_lanes) {
     const RayPre rp = precompute_ray(rays[lane]);
-    closest_hit_packet<false>(S, rp, true, h);
+    uint32_t steps = 0, tris = 0;
+    closest_hit_packet<false>(S, rp, true, h, steps, tris);
   }
   ids[lane] = (int32_t)h.id;
 }

@@ -145,6 +145,10 @@ def configure_extensions(lib):
     lib.setAdaptiveLabelError.restype = c_int
     lib.setAdaptiveSampler.argtypes = [c_int]
     lib.setAdaptiveSampler.restype = c_int
+    lib.setPacketCounting.argtypes = [c_bool]
+    lib.setPacketCounting.restype = None
+    lib.getPacketCounts.argtypes = [ndpointer(dtype=np.uint64, flags="C_CONTIGUOUS")]
+    lib.getPacketCounts.restype = c_int
     lib.getAdaptiveTimings.argtypes = [ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")]
     lib.getAdaptiveTimings.restype = c_int
     lib.setTraversalMode.argtypes = [c_int]
@@ -409,6 +413,14 @@ def render_adaptive_np(lib, poses6, error_target=0.01, max_spp=512, budget_scale
     if ms < 0:
         raise RuntimeError(last_error(lib))
     return ms, rgb, counts, int(total[0])
+
+
+def packet_counts(lib) -> dict:
+    """Packet node visits and leaf triangles counted since the last read (see setPacketCounting)."""
+    out = np.zeros(2, np.uint64)
+    if lib.getPacketCounts(out) != 0:
+        raise RuntimeError(last_error(lib))
+    return {"nodes": int(out[0]), "tris": int(out[1])}
 
 
 def adaptive_timings_np(lib) -> np.ndarray:

@@ -207,6 +207,15 @@ int nearestOmmatidia(const float* ommatidia, size_t count, int k, uint32_t* out)
 int64_t getAdaptiveStages(float* pilot, float* topup, uint32_t* extra, size_t maxUnits);
 /* Error the sampler network's labels were made for (default 0.01). */
 int setAdaptiveLabelError(float labelError);
+/* Measurement switch (no reference counterpart): while on, every wave-packet
+ * traversal of renderFrame, renderPoseBatch and renderPoseBatchAdaptive adds
+ * its node visits and the triangles of the leaves it enters, counted once
+ * per wave (what the wave fetches), to a device counter pair.  Off by
+ * default; the counting adds one atomic per wave. */
+void setPacketCounting(bool enabled);
+/* Reads the counter pair into out2 = (node visits, leaf triangles) and clears
+ * it.  Returns 0, or -1. */
+int getPacketCounts(uint64_t* out2);
 /* Where renderPoseBatchAdaptive gets each unit's required samples: 0 (the
  * default) the trained network; 1 the network-free estimate from the pooled
  * pilot standard deviation of the unit and its 6 nearest ommatidia, sigma =

@@ -6,6 +6,9 @@ natural_standin_sky.gltf.gz   data/natural-standin-sky.gltf, gzip-compressed:
                               simple_sky background, with four cameras.
 standin_eye.eye.gz            data/eyes/1000-horizontallyAcute-variableDegree.eye,
                               gzip-compressed (1,000 ommatidia).
+equidistant_eye.eye.gz        data/eyes/1000-equidistant.eye, gzip-compressed
+                              (1,000 ommatidia): the eye of the reference's
+                              speed test (python-examples/speed-test/speedTest.py:81).
 
 Run from the repository root: python tests/golden/make_standin_fixture.py
 """
@@ -21,6 +24,8 @@ def main() -> None:
     (HERE / "natural_standin_sky.gltf.gz").write_bytes(gzip.compress(gltf, 9, mtime=0))
     eye = (REF / "data" / "eyes" / "1000-horizontallyAcute-variableDegree.eye").read_bytes()
     (HERE / "standin_eye.eye.gz").write_bytes(gzip.compress(eye, 9, mtime=0))
+    eq = (REF / "data" / "eyes" / "1000-equidistant.eye").read_bytes()
+    (HERE / "equidistant_eye.eye.gz").write_bytes(gzip.compress(eq, 9, mtime=0))
 
 
 if __name__ == "__main__":

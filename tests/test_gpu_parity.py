@@ -503,3 +503,28 @@ def test_raw_ommatidial_samples_projection(lib, config1, oracle_c1, uneven):
         lib.setCurrentEyeShaderName(b"single_dimension_fast")
         er.set_sample_counts(lib, None)
         er.setRenderSize(lib, 400, 400)
+
+
+def test_packet_counts(lib, config1, oracle_c1):
+    """With counting on, renderFrame's packet traversal reports the nodes
+    its waves visit and the triangles of the leaves they enter.  Every lane
+    walks a superset of its own ray's nodes, so 64 x the packet visits bound
+    the rays' own visits (the oracle's per-ray count) from above; with
+    counting off nothing is added, and the colours are the same."""
+    omm = config1["eyes"][0][:640]
+    setup_eye(lib, config1, omm=omm, spp=64, record=False)
+    lib.setPacketCounting(False)
+    er.packet_counts(lib)
+    assert lib.renderFrame() >= 0
+    assert er.packet_counts(lib) == {"nodes": 0, "tris": 0}
+    lib.setCurrentEyeFrameIndex(0)
+    lib.setPacketCounting(True)
+    try:
+        assert lib.renderFrame() >= 0, er.last_error(lib)
+        pk = er.packet_counts(lib)
+    finally:
+        lib.setPacketCounting(False)
+    ref = oracle_c1.render(omm, er.pose_np(lib), spp=64, frame=0)
+    assert np.array_equal(er.colours_np(lib), ref["colour_gpu_order"][0])
+    assert 0 < pk["nodes"] and ref["nodes_visited"] <= 64 * pk["nodes"]
+    assert 0 < pk["tris"] and ref["tris_tested"] <= 64 * pk["tris"]
