@@ -32,6 +32,7 @@ SOURCES = [
     "mlp.hip",
     "adaptive.hip",
     "knn.cpp",
+    "noalloc.hip",
     "eye_renderer.cpp",
 ]
 

@@ -22,6 +22,7 @@
 #include "device_scene.h"
 #include "knn.h"
 #include "mlp.h"
+#include "noalloc.h"
 #include "render.h"
 #include "projection.h"
 #include "scene_loader.h"
@@ -168,6 +169,8 @@ void ensure_hip() {
   const hipError_t e = hipGetDeviceCount(&count);
   if (e != hipSuccess || count == 0) throw std::runtime_error("no HIP device available");
   hip_check(hipSetDevice(0), "hipSetDevice");
+  std::string serr;
+  if (!no_alloc_scratch_init(0, &serr)) throw std::runtime_error(serr);
   hip_check(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking), "hipStreamCreate");
   hip_check(hipMalloc(&s.d_pose, sizeof(Pose)), "hipMalloc pose");
   hip_check(hipMalloc(&s.d_stats, sizeof(unsigned long long) * 4), "hipMalloc stats");
@@ -1209,6 +1212,35 @@ double renderPoseBatch(const float* poses, size_t poseCount, float* rgbOut) {
   check_packet_fault("renderPoseBatch");
   return ms;
   ABI_GUARD_END(return -1.0)
+}
+
+int renderPoseBatchInto(const float* devPoses, size_t poseCount, float* devColours, void* stream) {
+  ABI_GUARD_BEGIN
+  State& s = st();
+  EyeState* e = current_eye();
+  if (!s.loaded || !e) throw std::runtime_error("renderPoseBatchInto: no compound eye selected");
+  if (poseCount == 0) return 0;
+  if (!devPoses || !devColours) throw std::runtime_error("renderPoseBatchInto: null device pointer");
+  if ((reinterpret_cast<uintptr_t>(devPoses) & 3u) != 0 || (reinterpret_cast<uintptr_t>(devColours) & 15u) != 0)
+    throw std::runtime_error("renderPoseBatchInto: poses must be 4-byte and colours 16-byte aligned");
+  const uint64_t units = (uint64_t)poseCount * (uint64_t)e->omms.size();
+  if (poseCount > (size_t)INT32_MAX || units > kNoAllocMaxUnits)
+    throw std::runtime_error("renderPoseBatchInto: " + std::to_string(units) + " units exceed the maximum of " +
+                             std::to_string(kNoAllocMaxUnits) + " per call");
+  if (e->device_dirty)
+    throw std::runtime_error(
+        "renderPoseBatchInto: the current eye's ommatidia or sample counts changed since they were last uploaded; "
+        "render the eye once with renderFrame or renderPoseBatch first (those calls may allocate)");
+  uint32_t* scratch = no_alloc_scratch(0);
+  if (!scratch) throw std::runtime_error("renderPoseBatchInto: fixed scratch not initialised");
+  UnitLaunch U = unit_launch_of(*e, reinterpret_cast<const Pose*>(devPoses), (int)poseCount);
+  U.colours = reinterpret_cast<float4*>(devColours);
+  U.queue = scratch;
+  std::string err;
+  if (!launch_render_units(s.scene, U, stream ? static_cast<hipStream_t>(stream) : s.stream, &err))
+    throw std::runtime_error(err);
+  return 0;
+  ABI_GUARD_END(return -1)
 }
 
 void setBatchAutotune(int enabled) { st().autotune = enabled != 0; }

@@ -131,6 +131,8 @@ def configure_extensions(lib):
     lib.probePacketGuard.restype = c_int
     lib.renderPoseBatch.argtypes = [f32p, c_size_t, c_void_p]
     lib.renderPoseBatch.restype = c_double
+    lib.renderPoseBatchInto.argtypes = [c_void_p, c_size_t, c_void_p, c_void_p]
+    lib.renderPoseBatchInto.restype = c_int
     lib.adaptiveTrainingSet.argtypes = [c_void_p, c_size_t, c_void_p, c_int, c_int, c_float, c_void_p, c_void_p,
                                         c_size_t]
     lib.adaptiveTrainingSet.restype = c_int64

@@ -169,6 +169,26 @@ int probePacketGuard(const float* rays64, int activeLanes, int32_t* ids64);
  * Pose k uses counter word 3 = k.  Returns GPU milliseconds, or -1 on error. */
 double renderPoseBatch(const float* poses, size_t poseCount, float* rgbOut);
 
+/* renderPoseBatch without any allocation.  No counterpart in the reference:
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).
+ * devPoses: poseCount device-resident poses of 12 floats each, the layout of
+ * getCameraPose (position, then the x, y and z axes; 4-byte aligned);
+ * devColours: device output, 4 floats per (pose, ommatidium), pose-major:
+ * (r, g, b, samples) (16-byte aligned).  The same units, random streams and
+ * colours as renderPoseBatch at the same poses.  The work is enqueued on
+ * `stream` (a hipStream_t; NULL: the library's stream) and not waited for.
+ * The unit renderer's work counters live in a fixed 256-byte device array
+ * (one per GPU, held until the process ends, cleared by a kernel at the start
+ * of every call); nothing is allocated and no hipMalloc/hipFree is called.
+ * Calls that overlap on one GPU race on that array.  Limits: at most
+ * 2^32 - 65537 (pose, ommatidium) units per call; the current eye must be
+ * resident, i.e. rendered by renderFrame or renderPoseBatch since its
+ * ommatidia or sample counts last changed (those calls upload it and may
+ * allocate).  A packet traversal fault is reported by the next call that
+ * synchronises.  Returns 0, or -1 with the error set. */
+int renderPoseBatchInto(const float* devPoses, size_t poseCount, float* devColours, void* stream);
+
 /* ---- Learned sampler MLP (16 -> 64 -> 64 -> 64 -> 1, BF16 on MFMA) ----
  * Parameters are FP32 in the order W1[16][64], b1[64], W2[64][64], b2[64],
  * W3[64][64], b3[64], W4[64], b4.  Host arrays; all return 0 or -1. */
