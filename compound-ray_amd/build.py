@@ -22,6 +22,7 @@ LIB_NAME = "libEyeRenderer3.so"
 
 SOURCES = [
     "json.cpp",
+    "jpeg.cpp",
     "png.cpp",
     "scene_loader.cpp",
     "scan.hip",

@@ -7,6 +7,7 @@
 #include <fstream>
 #include <sstream>
 
+#include "jpeg.h"
 #include "json.h"
 #include "png.h"
 
@@ -358,9 +359,13 @@ struct Loader {
         err.clear();
         return -1;
       }
-      if (!png_decode(bytes.data(), bytes.size(), &w, &h, &rgba, &perr)) {
+      const bool jpeg = is_jpeg(bytes.data(), bytes.size());
+      const bool ok = jpeg ? jpeg_decode(bytes.data(), bytes.size(), &w, &h, &rgba, &perr)
+                           : png_decode(bytes.data(), bytes.size(), &w, &h, &rgba, &perr);
+      if (!ok) {
         out->warnings.push_back("image " + std::to_string(src) + ": " + perr +
-                                " (only PNG images are decoded); texture " + std::to_string(t) + " not sampled");
+                                (jpeg ? "" : " (PNG and JPEG images are decoded)") + "; texture " +
+                                std::to_string(t) + " not sampled");
         return -1;
       }
       if (out->texels.size() + (size_t)w * (size_t)h > 0x7fffffffu) {

@@ -1,12 +1,13 @@
 // AddressSanitizer + UndefinedBehaviorSanitizer driver for the host-side code
 // that reads untrusted files or runs on the CPU only: the glTF/JSON reader,
-// the PNG/DEFLATE decoder, the .eye reader, the neighbour table and the CPU
+// the PNG/DEFLATE and JPEG decoders, the .eye reader, the neighbour table and the CPU
 // oracle.  Test infrastructure (built and run by tests/test_sanitizers.py);
 // malformed inputs are expected to be rejected with an error, never to read
 // or write out of bounds or hit undefined behaviour.
 //
 //   driver gltf FILE...        load_gltf on each file
 //   driver png FILE...         png_decode on each file
+//   driver jpeg FILE...        jpeg_decode on each file
 //   driver eye FILE...         read_eye_file on each file
 //   driver knn N K SEED        nearest_ommatidia on N seeded ommatidia
 //   driver render FILE SPP     load_gltf, then the oracle's BVH build and a
@@ -21,6 +22,7 @@
 #include <string>
 #include <vector>
 
+#include "jpeg.h"
 #include "knn.h"
 #include "png.h"
 #include "scene_loader.h"
@@ -68,6 +70,20 @@ int cmd_png(int argc, char** argv) {
     const bool ok = eye::png_decode(bytes.data(), bytes.size(), &w, &h, &rgba, &err);
     unsigned sum = 0;
     for (uint8_t v : rgba) sum += v;   // touches every decoded byte
+    std::printf("%s: %s %dx%d sum %u\n", argv[i], ok ? "ok" : err.c_str(), w, h, sum);
+  }
+  return 0;
+}
+
+int cmd_jpeg(int argc, char** argv) {
+  for (int i = 0; i < argc; ++i) {
+    std::vector<uint8_t> bytes, rgba;
+    if (!read_all(argv[i], &bytes)) return 2;
+    int w = 0, h = 0;
+    std::string err;
+    const bool ok = eye::jpeg_decode(bytes.data(), bytes.size(), &w, &h, &rgba, &err);
+    unsigned sum = 0;
+    for (uint8_t v : rgba) sum += v;
     std::printf("%s: %s %dx%d sum %u\n", argv[i], ok ? "ok" : err.c_str(), w, h, sum);
   }
   return 0;
@@ -186,6 +202,7 @@ int main(int argc, char** argv) {
   const std::string cmd = argv[1];
   if (cmd == "gltf") return cmd_gltf(argc - 2, argv + 2);
   if (cmd == "png") return cmd_png(argc - 2, argv + 2);
+  if (cmd == "jpeg") return cmd_jpeg(argc - 2, argv + 2);
   if (cmd == "eye") return cmd_eye(argc - 2, argv + 2);
   if (cmd == "knn") return cmd_knn(argc - 2, argv + 2);
   if (cmd == "render") return cmd_render(argc - 2, argv + 2);

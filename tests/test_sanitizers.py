@@ -1,10 +1,11 @@
 """AddressSanitizer + UndefinedBehaviorSanitizer run of the host code that
-reads untrusted files (glTF/JSON, PNG/DEFLATE, .eye), the neighbour table and
+reads untrusted files (glTF/JSON, PNG/DEFLATE, JPEG, .eye), the neighbour table and
 the CPU oracle (CPU only; tests/sanitize/).  Well-formed inputs, the
 reference's own scene files, and seeded truncations and byte corruptions of
 them must load or be rejected with an error, never touch memory out of
 bounds, leak, or hit undefined behaviour."""
 import gzip
+import io
 import os
 import pathlib
 import shutil
@@ -14,7 +15,10 @@ import numpy as np
 import pytest
 
 from compound_ray_amd import scenes
+from jpeg_writer import write_jpeg
 from test_textures import write_png
+
+Image = pytest.importorskip("PIL.Image")
 
 HERE = pathlib.Path(__file__).resolve().parent
 SAN_DIR = HERE / "sanitize"
@@ -82,6 +86,23 @@ def corpus(tmp_path_factory):
         p.write_bytes(write_png(px, ctype, depth, interlace=k % 2 == 1, palette=pal))
         pngs.append(p)
     files["pngs"] = pngs
+    jpegs = []
+    img = rng.integers(0, 256, size=(19, 27, 3)).astype(np.uint8)
+    for k, kw in enumerate([dict(subsampling=2), dict(subsampling=1, restart_marker_blocks=2), dict(subsampling=0),
+                            dict(progressive=True)]):
+        b = io.BytesIO()
+        Image.fromarray(img).save(b, "JPEG", quality=85, **kw)
+        p = d / f"img{k}.jpg"
+        p.write_bytes(b.getvalue())
+        jpegs.append(p)
+    q = {0: rng.integers(1, 400, 64)}
+    comp = [dict(id=1, h=2, v=2, tq=0, coef=rng.integers(-5, 5, size=(4, 4, 64))),
+            dict(id=2, h=1, v=1, tq=0, coef=rng.integers(-5, 5, size=(2, 2, 64))),
+            dict(id=3, h=1, v=1, tq=0, coef=rng.integers(-5, 5, size=(2, 2, 64)))]
+    p = d / "writer.jpg"
+    p.write_bytes(write_jpeg(30, 25, comp, q, [[0], [1, 2]], restart=1, extended=True))
+    jpegs.append(p)
+    files["jpegs"] = jpegs
     return d, files
 
 
@@ -91,6 +112,8 @@ def test_parsers_on_well_formed_files(driver, corpus):
     assert out.count(": ok (") == 3
     out = run(driver, "png", *files["pngs"])
     assert out.count(": ok ") == len(files["pngs"])
+    out = run(driver, "jpeg", *files["jpegs"])
+    assert out.count(": ok ") == len(files["jpegs"]) - 1 and "progressive JPEG is not supported" in out
     out = run(driver, "eye", *files["eyes"])
     assert "(1000 ommatidia)" in out and "(100 ommatidia)" in out
 
@@ -115,6 +138,13 @@ def test_parsers_on_corrupted_files(driver, corpus, tmp_path):
         p.write_bytes(m)
         cases.append(p)
     run(driver, "png", *cases)
+    cases = []
+    for j, src in enumerate((files["jpegs"][0], files["jpegs"][1], files["jpegs"][4])):
+        for k, m in enumerate(mutants(src.read_bytes(), 10 + j, 80)):
+            p = tmp_path / f"jpeg{j}_{k}.jpg"
+            p.write_bytes(m)
+            cases.append(p)
+    run(driver, "jpeg", *cases)
     # glTF documents with an embedded buffer: JSON and accessor corruption
     gltf = files["test_scene"].read_bytes()
     cases = []
