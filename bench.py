@@ -363,15 +363,20 @@ def cpu_model() -> str:
     return name
 
 
-def traversal_bytes(st: dict, units: int, per_unit: int) -> float:
-    """Algorithmic bytes of traced rays: BVH nodes visited x 128 B + triangles
+def node_bytes(lib, er) -> float:
+    """Bytes of one BVH node (read whole at every visit) of the loaded scene."""
+    return float(er.bvh_info(lib)["node_bytes"])
+
+
+def traversal_bytes(st: dict, units: int, per_unit: int, nbytes: float) -> float:
+    """Algorithmic bytes of traced rays: BVH nodes visited x `nbytes` + triangles
     tested x 48 B + shading fetch (3 x 16 B) per hit, plus `per_unit` bytes of
     per-unit records (ommatidium read, colour and statistics written)."""
-    return st["nodes_visited"] * 128.0 + st["tris_tested"] * 48.0 + st["hits"] * 48.0 + units * float(per_unit)
+    return st["nodes_visited"] * nbytes + st["tris_tested"] * 48.0 + st["hits"] * 48.0 + units * float(per_unit)
 
 
 def bench_large(lib, er, scenes, tmp: str, steps: int, warmup: int, target: float = 0.01,
-                ref_per_eye: int = 400, cpu_per_eye: int = 20, cpu: bool = True) -> dict:
+                ref_per_eye: int = 400, cpu_per_eye: int = 20, cpu: bool = True, profile: bool = False) -> dict:
     """The headline: 256 eyes (log-uniform 1k-30k ommatidia, Fibonacci-sphere
     and ellipsoidal layouts, acceptance 1-5 deg), each at its own pose over a
     16.3M-triangle scene, rendered as one batch (all eyes' ommatidia moved to
@@ -489,6 +494,15 @@ def bench_large(lib, er, scenes, tmp: str, steps: int, warmup: int, target: floa
         stage.append(er.adaptive_timings_np(lib))
     elapsed = time.perf_counter() - t0
     stage = np.array(stage)
+    if profile:
+        # profiler runs: the timed adaptive renders are the last launches
+        return {"_timed": {"elapsed": elapsed, "steps": steps, "frames_per_step": len(eyes),
+                           "samples_per_step": total},
+                "workload": "profile run", "bvh": bvh, "grays_per_s_adaptive": total / (elapsed / steps) / 1e9,
+                "gain_vs_uniform_batched": None, "gain_vs_network_free": None, "rms_abs_adaptive": e_a,
+                "stage_ms": {"pilot": float(stage[:, 0].mean()), "allocation": float(stage[:, 1].mean()),
+                             "top_up": float(stage[:, 2].mean()), "combine": float(stage[:, 3].mean())},
+                "roofline": None}
 
     def timed(fn, repeats=3):
         fn()
@@ -517,7 +531,8 @@ def bench_large(lib, er, scenes, tmp: str, steps: int, warmup: int, target: floa
     # pilot: ommatidium 32 B read, colour 16 B + statistics 32 B written per
     # unit; top-up: ommatidium 32 B + count 4 B read, colour 16 B written
     unit_io = n_omm * ((32 + 16 + 32) + (32 + 4 + 16))
-    b_per_ray = traversal_bytes(st_pilot, 0, 0) + traversal_bytes(st_top, 0, 0) + unit_io
+    nb = node_bytes(lib, er)
+    b_per_ray = traversal_bytes(st_pilot, 0, 0, nb) + traversal_bytes(st_top, 0, 0, nb) + unit_io
     # what the waves fetch: a packet reads each node it visits and the
     # triangles of each leaf it enters once for its 64 rays (counted on the
     # GPU in an untimed run of the same render); shading gathers stay per ray
@@ -526,7 +541,7 @@ def bench_large(lib, er, scenes, tmp: str, steps: int, warmup: int, target: floa
     adaptive(scale, want_rgb=False)
     pk = er.packet_counts(lib)
     lib.setPacketCounting(False)
-    b_packet = pk["nodes"] * 128.0 + pk["tris"] * 48.0 + st_all["hits"] * 48.0 + unit_io
+    b_packet = pk["nodes"] * nb + pk["tris"] * 48.0 + st_all["hits"] * 48.0 + unit_io
     ms_pilot = float(stage[:, 0].mean())
     ms_top = float(stage[:, 2].mean())
     launch_ms = (ms_pilot + ms_top) / 2
@@ -563,8 +578,8 @@ def bench_large(lib, er, scenes, tmp: str, steps: int, warmup: int, target: floa
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": "k_render_units",
                      "bytes_per_launch": bytes_launch, "launch_ms": launch_ms,
                      "launches": "pilot and top-up of one adaptive render (HIP events on the library stream)",
-                     "bytes": "per wave: packet node visits x 128 B + leaf triangles x 48 B; per hit 48 B of "
-                              "shading data; per unit 132 B of ommatidium, count, colour and statistics",
+                     "bytes": f"per wave: packet node visits x {nb:.0f} B (one node) + leaf triangles x 48 B; per hit "
+                              "48 B of shading data; per unit 132 B of ommatidium, count, colour and statistics",
                      "packet_nodes_per_64_rays": pk["nodes"] * 64.0 / total,
                      "packet_tris_per_64_rays": pk["tris"] * 64.0 / total,
                      "per_ray_equivalent_GBps": b_per_ray / 2 / (launch_ms * 1e-3) / 1e9,
@@ -715,10 +730,11 @@ def bench_plain(lib, er, cfg: dict, eyes: list, steps: int, warmup: int, trace_m
     trace_avg_ms = float(np.mean(trace_ms))
     # algorithmic bytes of one render launch (one eye-frame), per ray as the
     # rays' own traversals count them (a packet fetches a node once per wave)
-    bytes_launch = traversal_bytes(stats, omm_total, 32 + 16) / len(eyes)
+    nb = node_bytes(lib, er)
+    bytes_launch = traversal_bytes(stats, omm_total, 32 + 16, nb) / len(eyes)
     achieved = bytes_launch / (trace_avg_ms * 1e-3) / 1e9
     # what the waves fetch (each node and leaf once per 64-ray packet)
-    packet_launch = (pk["nodes"] * 128.0 + pk["tris"] * 48.0 + stats["hits"] * 48.0 + omm_total * 48.0) / len(eyes)
+    packet_launch = (pk["nodes"] * nb + pk["tris"] * 48.0 + stats["hits"] * 48.0 + omm_total * 48.0) / len(eyes)
     traffic = measured_traffic(kernel_name, "c2")
     if copy_probe > 0:
         nbytes = int(copy_probe * 2 ** 20)
@@ -830,7 +846,7 @@ def main() -> int:
     if not (only or args.skip_large):
         log("large heterogeneous batch (headline) ...")
         large = bench_large(lib, er, scenes, tmp, args.steps, args.warmup,
-                            cpu=not (args.no_cpu_baseline or args.profile))
+                            cpu=not (args.no_cpu_baseline or args.profile), profile=args.profile)
         timed = large.pop("_timed")
         out["value"] = timed["frames_per_step"] * timed["steps"] / timed["elapsed"]
         out["ms_per_step"] = timed["elapsed"] / timed["steps"] * 1e3
@@ -850,9 +866,12 @@ def main() -> int:
         if "cpu_baseline" in large:
             out["cpu_baseline"] = large.pop("cpu_baseline")
         out["large_batch"] = large
-        log(f"C5: {out['value']:.0f} eye-frames/s adaptive, gain {large['gain_vs_uniform_batched']:.2f}x over "
-            f"uniform, {large['gain_vs_network_free']:.2f}x over network-free")
+        if not args.profile:
+            log(f"C5: {out['value']:.0f} eye-frames/s adaptive, gain {large['gain_vs_uniform_batched']:.2f}x over "
+                f"uniform, {large['gain_vs_network_free']:.2f}x over network-free")
     if args.profile and not only:
+        if args.copy_probe > 0:
+            lib.runCopyProbe(int(args.copy_probe * 2 ** 20), 10)
         lib.stop()
         print(json.dumps(out))
         return 0

@@ -12,13 +12,13 @@
 //      of BVHs, octrees, and k-d trees", HPG 2012) is built over the sorted
 //      codes, ties between equal codes broken by index;
 //   4. bounds are refitted bottom-up (min/max, order independent, exact);
-//   5. the binary tree is collapsed top-down into 4-wide nodes: starting from
-//      a node's two children, the internal child with the largest surface area
-//      (first in slot order on ties) is opened until four children exist or
-//      none can be opened; any binary node covering at most `leaf_max`
-//      triangles becomes a leaf that references its contiguous range of
-//      sorted triangles.  Wide nodes are numbered breadth-first, level by
-//      level, in parent order then slot order.
+//   5. the binary tree is collapsed top-down into kBvhWidth-wide nodes:
+//      starting from a node's two children, the internal child with the
+//      largest surface area (first in slot order on ties) is opened until
+//      kBvhWidth children exist or none can be opened; any binary node
+//      covering at most `leaf_max` triangles becomes a leaf that references
+//      its contiguous range of sorted triangles.  Wide nodes are numbered
+//      breadth-first, level by level, in parent order then slot order.
 // Every rule is deterministic, so a CPU build of the same codes produces the
 // same node array byte for byte.
 #pragma once
@@ -28,23 +28,39 @@
 
 namespace eye {
 
+// Children per wide node: 4 or 8 (a multiple of kHalfWidth).  The build,
+// the traversals and the CPU oracle all follow this constant.  Measured on
+// MI355X: an 8-wide tree cut the packet traversal's node visits by a third
+// (44.7 against 66.1 per 64 rays at bee scale) but, at twice the bytes and
+// slab tests per visit, rendered 2-5 % slower on every workload, so the tree
+// is 4-wide.
 constexpr int kBvhWidth = 4;
+constexpr int kHalfWidth = 4;                        // child slots per 128-byte half
+constexpr int kBvhHalves = kBvhWidth / kHalfWidth;
+static_assert(kBvhWidth % kHalfWidth == 0, "whole node halves");
 constexpr int kLeafMaxLimit = 16;
 constexpr int32_t kChildEmpty = 0x7fffffff;
 // Far-away degenerate box for empty child slots: never intersected by a ray
 // with |origin| < 1e30 and tmax <= 1e16.
 constexpr float kEmptyBox = 3.0e38f;
 
-// 128-byte wide node: structure-of-arrays over the four child slots so that a
-// lane reads it as eight aligned 16-byte vectors.
-struct alignas(16) WideNode {
+// 128 bytes holding four child slots as structure-of-arrays, so that a lane
+// (or the scalar unit) reads them as eight aligned 16-byte vectors.
+struct alignas(16) WideHalf {
   float lo_x[4], hi_x[4];
   float lo_y[4], hi_y[4];
   float lo_z[4], hi_z[4];
   int32_t child[4];   // >= 0 internal wide node; < 0 leaf; kChildEmpty unused
   uint32_t pad[4];
 };
-static_assert(sizeof(WideNode) == 128, "wide node must be 128 bytes");
+static_assert(sizeof(WideHalf) == 128, "a node half must be 128 bytes");
+
+// Wide node: child slot s lives in half s / 4 at position s % 4.  Slots fill
+// in order; empty slots hold kChildEmpty and the far box.
+struct alignas(16) WideNode {
+  WideHalf half[kBvhHalves];
+};
+static_assert(sizeof(WideNode) == 128 * kBvhHalves, "wide node layout");
 
 // Leaf child encoding: bit 31 set, bits 27..30 = count-1, bits 0..26 = first
 // sorted triangle.
@@ -149,9 +165,9 @@ EYE_HD int bin_size(const BinaryTreeView& t, int node) {
 }
 EYE_HD bool bin_is_leaf(const BinaryTreeView& t, int node) { return node >= t.n - 1; }
 
-// Children of the wide node built from binary node `src`: fills up to four
+// Children of the wide node built from binary node `src`: fills up to eight
 // combined binary indices, returns the count.
-EYE_HD int collapse_children(const BinaryTreeView& t, int src, int leaf_max, int out[4]) {
+EYE_HD int collapse_children(const BinaryTreeView& t, int src, int leaf_max, int out[kBvhWidth]) {
   int cnt;
   if (bin_is_leaf(t, src) || bin_size(t, src) <= leaf_max) {
     out[0] = src;

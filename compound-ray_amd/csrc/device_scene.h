@@ -1,4 +1,4 @@
-// Device-resident scene: the 4-wide BVH, the triangles and their shading data
+// Device-resident scene: the wide BVH, the triangles and their shading data
 // in BVH (Morton) order.  Host-side declarations shared by the C ABI, the
 // LBVH builder and the render kernels.
 #pragma once

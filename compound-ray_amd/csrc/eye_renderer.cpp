@@ -405,7 +405,7 @@ double render_compound(EyeState& e, const Cam& c) {
     U.samples = e.d_samples;
     U.sample_offsets = e.d_sample_offsets;
   }
-  if (e.counts.empty() && e.spp > 64) {
+  if (e.counts.empty()) {
     // one pose: let the chunks of each ommatidium's samples spread over waves
     const size_t chunks = ((size_t)e.spp + 63) / 64;
     ensure_device(&e.d_chunk_partial, &e.d_chunk_partial_cap, e.omms.size() * chunks * 64);
@@ -1499,7 +1499,7 @@ int getSceneBVHInfo(int64_t* info) {
   info[3] = s.scene.levels;
   info[4] = s.scene.leaf_max;
   info[5] = (int64_t)(s.scene.build_ms * 1000.0);
-  info[6] = 0;
+  info[6] = (int64_t)sizeof(WideNode);
   info[7] = 0;
   return 0;
 }

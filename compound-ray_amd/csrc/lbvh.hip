@@ -1,5 +1,5 @@
 // On-GPU LBVH build for gfx950: Morton codes, LDS-staged stable radix sort,
-// Karras hierarchy, bottom-up refit, breadth-first collapse to a 4-wide BVH.
+// Karras hierarchy, bottom-up refit, breadth-first collapse to a kBvhWidth-wide BVH.
 // The rules are the ones documented in bvh_format.h; this file is the
 // parallel implementation.  Replaces the OptiX GAS/IAS build of
 // libEyeRenderer3/MulticamScene.cpp:1052-1428.
@@ -294,7 +294,7 @@ __global__ void k_collapse_count(const int32_t* __restrict__ srcs, int count, Bi
                                  int leaf_max, uint32_t* __restrict__ internal_count) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= count) return;
-  int ch[4];
+  int ch[kBvhWidth];
   const int c = collapse_children(t, srcs[j], leaf_max, ch);
   uint32_t k = 0;
   for (int s = 0; s < c; ++s) k += wide_child_is_internal(t, ch[s], leaf_max) ? 1u : 0u;
@@ -308,7 +308,7 @@ __global__ void k_collapse_emit(const int32_t* __restrict__ srcs, int count, Bin
                                 int32_t* __restrict__ stack_bound) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= count) return;
-  int ch[4];
+  int ch[kBvhWidth];
   const int c = collapse_children(t, srcs[j], leaf_max, ch);
   const int w = base + j;
   const int dw = depth[w];
@@ -316,28 +316,30 @@ __global__ void k_collapse_emit(const int32_t* __restrict__ srcs, int count, Bin
   for (int s = 0; s < c; ++s) k_internal += wide_child_is_internal(t, ch[s], leaf_max) ? 1 : 0;
   WideNode node;
   int next = 0;
-  for (int s = 0; s < 4; ++s) {
+  for (int s = 0; s < kBvhWidth; ++s) {
+    WideHalf& hf = node.half[s / kHalfWidth];
+    const int q = s % kHalfWidth;
     if (s < c) {
       const int b = ch[s];
       const float* bb = t.bounds + 6 * (size_t)b;
-      node.lo_x[s] = bb[0]; node.lo_y[s] = bb[1]; node.lo_z[s] = bb[2];
-      node.hi_x[s] = bb[3]; node.hi_y[s] = bb[4]; node.hi_z[s] = bb[5];
+      hf.lo_x[q] = bb[0]; hf.lo_y[q] = bb[1]; hf.lo_z[q] = bb[2];
+      hf.hi_x[q] = bb[3]; hf.hi_y[q] = bb[4]; hf.hi_z[q] = bb[5];
       if (wide_child_is_internal(t, b, leaf_max)) {
         const int slot = (int)offs[j] + next;
         ++next;
-        node.child[s] = next_base + slot;
+        hf.child[q] = next_base + slot;
         next_srcs[slot] = b;
         depth[next_base + slot] = dw + (k_internal - 1);
       } else {
-        node.child[s] = encode_leaf((uint32_t)t.range[2 * (size_t)b], (uint32_t)bin_size(t, b));
+        hf.child[q] = encode_leaf((uint32_t)t.range[2 * (size_t)b], (uint32_t)bin_size(t, b));
       }
     } else {
-      node.lo_x[s] = node.hi_x[s] = kEmptyBox;
-      node.lo_y[s] = node.hi_y[s] = kEmptyBox;
-      node.lo_z[s] = node.hi_z[s] = kEmptyBox;
-      node.child[s] = kChildEmpty;
+      hf.lo_x[q] = hf.hi_x[q] = kEmptyBox;
+      hf.lo_y[q] = hf.hi_y[q] = kEmptyBox;
+      hf.lo_z[q] = hf.hi_z[q] = kEmptyBox;
+      hf.child[q] = kChildEmpty;
     }
-    node.pad[s] = 0;
+    hf.pad[q] = 0;
   }
   nodes[w] = node;
   atomicMax(stack_bound, dw + (k_internal > 1 ? k_internal - 1 : 0));
@@ -388,8 +390,8 @@ __global__ void k_count_refs(const WideNode* __restrict__ nodes, int node_count,
                              uint32_t* __restrict__ covered, uint32_t* __restrict__ flags) {
   const int w = blockIdx.x * blockDim.x + threadIdx.x;
   if (w >= node_count) return;
-  for (int s = 0; s < 4; ++s) {
-    const int32_t c = nodes[w].child[s];
+  for (int s = 0; s < kBvhWidth; ++s) {
+    const int32_t c = nodes[w].half[s / kHalfWidth].child[s % kHalfWidth];
     if (c == kChildEmpty) continue;
     if (child_is_leaf(c)) {
       const uint32_t f = leaf_first(c), k = leaf_count(c);

@@ -49,8 +49,8 @@ struct UnitLaunch {
   float4* samples = nullptr;            // nullable: per-sample (r, g, b, t or -1)
   const uint64_t* sample_offsets = nullptr;  // first sample record of each unit
   uint32_t* queue = nullptr;            // device work counters (kQueueWords)
-  // Optional scratch for split launches (uniform sample counts above 64 and
-  // few units): each unit's 64-sample chunks go to different waves; per-lane
+  // Optional scratch for split launches (uniform sample counts and few
+  // units): each unit's 64-sample chunks go to different waves; per-lane
   // chunk values (1 float4, 3 with stats) and a completion counter per unit.
   float4* chunk_partial = nullptr;
   uint64_t chunk_partial_cap = 0;       // float4 elements

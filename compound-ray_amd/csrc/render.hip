@@ -1,4 +1,4 @@
-// gfx950 render kernels: fused Gaussian-cone ray generation + 4-wide BVH
+// gfx950 render kernels: fused Gaussian-cone ray generation + wide BVH
 // traversal + shading (replacing __raygen__ommatidium, the OptiX traversal,
 // __closesthit__radiance and the miss programs, libEyeRenderer3/shaders.cu:664-811),
 // and the per-ommatidium wave reduction + projection (shaders.cu:341-406).
@@ -138,9 +138,33 @@ __device__ __forceinline__ void test_leaf_wave(const float4* tris, int32_t code,
   }
 }
 
-// Closest hit over the 4-wide BVH.  `stack` points at this lane's column of
+// Inserts (key, child) into the list of the first n entries of (keys,
+// children), kept in increasing key order; among equal keys the earlier
+// insertion stays first.  Every index is a compile-time constant, so the
+// lists stay in registers (scalar registers when the values are
+// wave-uniform): a list indexed by a run-time count would be placed in scratch.
+template <typename K>
+__device__ __forceinline__ void sorted_insert(K (&keys)[kBvhWidth], int (&children)[kBvhWidth], int n, K key,
+                                              int child) {
+  int pos = 0;
+#pragma unroll
+  for (int i = 0; i < kBvhWidth; ++i) pos += (i < n && !(key < keys[i])) ? 1 : 0;
+#pragma unroll
+  for (int j = kBvhWidth - 1; j >= 0; --j) {
+    const K below_k = j > 0 ? keys[j - 1] : key;
+    const int below_c = j > 0 ? children[j - 1] : child;
+    keys[j] = j < pos ? keys[j] : (j == pos ? key : below_k);
+    children[j] = j < pos ? children[j] : (j == pos ? child : below_c);
+  }
+}
+
+// Closest hit over the kBvhWidth-wide BVH.  `stack` points at this lane's column of
 // an LDS array laid out [depth][blockDim.x].  The result is independent of
-// the traversal order: the (t, id) pair is minimised lexicographically.
+// the traversal order: the (t, id) pair is minimised lexicographically.  At a
+// node, the entry distances of all child boxes use the culling limit of
+// the hit found before the node; the slots are taken in order: leaves are
+// tested at once, internal children are visited nearest first (equal
+// distances in slot order).
 template <bool kStats, bool kParallel>
 __device__ __forceinline__ void closest_hit_impl(const SceneView& S, const RayPre& rp, uint32_t* stack,
                                                  int stride, Hit& h, uint32_t* nodes_visited,
@@ -155,29 +179,32 @@ __device__ __forceinline__ void closest_hit_impl(const SceneView& S, const RayPr
   for (;;) {
     if (kStats) ++*nodes_visited;
     const float4* np = reinterpret_cast<const float4*>(S.nodes + node);
-    const float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5];
-    const int4 ch = reinterpret_cast<const int4*>(np)[6];
-    float te[4];
     const float limit = h.t * kCullSlack;
-    te[0] = slab_enter<kParallel>(rp, lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, limit);
-    te[1] = slab_enter<kParallel>(rp, lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, limit);
-    te[2] = slab_enter<kParallel>(rp, lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, limit);
-    te[3] = slab_enter<kParallel>(rp, lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, limit);
-    const int32_t cc[4] = {ch.x, ch.y, ch.z, ch.w};
     int ni = 0;
-    int ic[4];
-    float it[4];
+    int ic[kBvhWidth];
+    float it[kBvhWidth];
 #pragma unroll
-    for (int s = 0; s < 4; ++s) {
-      const int32_t c = cc[s];
-      if (c == kChildEmpty || !(te[s] < INFINITY)) continue;
-      if (child_is_leaf(c)) {
-        if (kStats) *tris_tested += leaf_count(c);
-        test_leaf(VaryingPtr{S.tris}, c, rp, h);
-      } else {
-        ic[ni] = c;
-        it[ni] = te[s];
-        ++ni;
+    for (int hf = 0; hf < kBvhHalves; ++hf) {
+      const float4* q = np + 8 * hf;
+      const float4 lx = q[0], hx = q[1], ly = q[2], hy = q[3], lz = q[4], hz = q[5];
+      const int4 ch = reinterpret_cast<const int4*>(q)[6];
+      float te[kHalfWidth];
+      te[0] = slab_enter<kParallel>(rp, lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, limit);
+      te[1] = slab_enter<kParallel>(rp, lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, limit);
+      te[2] = slab_enter<kParallel>(rp, lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, limit);
+      te[3] = slab_enter<kParallel>(rp, lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, limit);
+      const int32_t cc[kHalfWidth] = {ch.x, ch.y, ch.z, ch.w};
+#pragma unroll
+      for (int s = 0; s < kHalfWidth; ++s) {
+        const int32_t c = cc[s];
+        if (c == kChildEmpty || !(te[s] < INFINITY)) continue;
+        if (child_is_leaf(c)) {
+          if (kStats) *tris_tested += leaf_count(c);
+          test_leaf(VaryingPtr{S.tris}, c, rp, h);
+        } else {
+          sorted_insert(it, ic, ni, te[s], c);
+          ++ni;
+        }
       }
     }
     if (ni == 0) {
@@ -186,22 +213,14 @@ __device__ __forceinline__ void closest_hit_impl(const SceneView& S, const RayPr
       node = (int)stack[sp * stride];
       continue;
     }
-    // order internal children near to far (insertion sort, at most 4)
-    for (int a = 1; a < ni; ++a) {
-      const float tv = it[a];
-      const int cv = ic[a];
-      int b = a - 1;
-      while (b >= 0 && it[b] > tv) {
-        it[b + 1] = it[b];
-        ic[b + 1] = ic[b];
-        --b;
+    // internal children are in near-to-far order: push all but the nearest,
+    // farthest first, and visit the nearest
+#pragma unroll
+    for (int a = kBvhWidth - 1; a >= 1; --a) {
+      if (a < ni) {
+        stack[sp * stride] = (uint32_t)ic[a];
+        ++sp;
       }
-      it[b + 1] = tv;
-      ic[b + 1] = cv;
-    }
-    for (int a = ni - 1; a >= 1; --a) {
-      stack[sp * stride] = (uint32_t)ic[a];
-      ++sp;
     }
     node = ic[0];
   }
@@ -291,45 +310,51 @@ __device__ __forceinline__ int stack_top(int s0, int s1, int sp) {
 // ic / ik (child, ordering key).
 template <bool kParallel, bool kScalar>
 __device__ __forceinline__ void packet_visit(const SceneView& S, const RayPre& rp, bool active, int node, Hit& h,
-                                             int* ic, uint32_t* ik, int& ni, uint32_t& tris) {
-  float4 lx, hx, ly, hy, lz, hz;
-  int4 ch;
-  if (kScalar) {
-    const UniformPtr np{S.nodes + node};
-    lx = load4(np, 0); hx = load4(np, 1); ly = load4(np, 2);
-    hy = load4(np, 3); lz = load4(np, 4); hz = load4(np, 5);
-    ch = load_int4_uniform(np.p, 6);
-  } else {
-    const float4* np = reinterpret_cast<const float4*>(S.nodes + node);
-    lx = np[0]; hx = np[1]; ly = np[2]; hy = np[3]; lz = np[4]; hz = np[5];
-    ch = reinterpret_cast<const int4*>(np)[6];
-  }
+                                             int (&ic)[kBvhWidth], uint32_t (&ik)[kBvhWidth], int& ni,
+                                             uint32_t& tris) {
+  // every slot's entry distance uses the culling limit of the hit found
+  // before the node, also after the first half's leaves have been tested
   const float limit = h.t * kCullSlack;
-  float te[4];
-  te[0] = slab_enter<kParallel>(rp, lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, limit);
-  te[1] = slab_enter<kParallel>(rp, lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, limit);
-  te[2] = slab_enter<kParallel>(rp, lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, limit);
-  te[3] = slab_enter<kParallel>(rp, lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, limit);
-  const int32_t cc[4] = {ch.x, ch.y, ch.z, ch.w};
 #pragma unroll
-  for (int s = 0; s < 4; ++s) {
-    const int32_t c = cc[s];
-    const bool mine = active && te[s] < INFINITY;
-    if (c == kChildEmpty || __ballot(mine) == 0ull) continue;
-    if (child_is_leaf(c)) {
-      tris += leaf_count(c);
-      // inactive lanes test a dummy ray whose result is discarded
-      if (kScalar)
-        test_leaf(UniformPtr{S.tris}, c, rp, h);
-      else
-        test_leaf_wave(S.tris, c, rp, h);
+  for (int hf = 0; hf < kBvhHalves; ++hf) {
+    float4 lx, hx, ly, hy, lz, hz;
+    int4 ch;
+    if (kScalar) {
+      const UniformPtr np{S.nodes + node};
+      lx = load4(np, 8 * hf + 0); hx = load4(np, 8 * hf + 1); ly = load4(np, 8 * hf + 2);
+      hy = load4(np, 8 * hf + 3); lz = load4(np, 8 * hf + 4); hz = load4(np, 8 * hf + 5);
+      ch = load_int4_uniform(np.p, 8 * hf + 6);
     } else {
-      // order key: the entry distance of the first lane that enters the child
-      const uint64_t m = __ballot(mine);
-      const int lead = __builtin_amdgcn_readfirstlane(__ffsll((long long)m) - 1);
-      ic[ni] = c;
-      ik[ni] = (uint32_t)__builtin_amdgcn_readlane((int)__float_as_uint(te[s]), lead);
-      ++ni;
+      const float4* np = reinterpret_cast<const float4*>(S.nodes + node) + 8 * hf;
+      lx = np[0]; hx = np[1]; ly = np[2]; hy = np[3]; lz = np[4]; hz = np[5];
+      ch = reinterpret_cast<const int4*>(np)[6];
+    }
+    float te[kHalfWidth];
+    te[0] = slab_enter<kParallel>(rp, lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, limit);
+    te[1] = slab_enter<kParallel>(rp, lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, limit);
+    te[2] = slab_enter<kParallel>(rp, lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, limit);
+    te[3] = slab_enter<kParallel>(rp, lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, limit);
+    const int32_t cc[kHalfWidth] = {ch.x, ch.y, ch.z, ch.w};
+#pragma unroll
+    for (int s = 0; s < kHalfWidth; ++s) {
+      const int32_t c = cc[s];
+      const bool mine = active && te[s] < INFINITY;
+      if (c == kChildEmpty || __ballot(mine) == 0ull) continue;
+      if (child_is_leaf(c)) {
+        tris += leaf_count(c);
+        // inactive lanes test a dummy ray whose result is discarded
+        if (kScalar)
+          test_leaf(UniformPtr{S.tris}, c, rp, h);
+        else
+          test_leaf_wave(S.tris, c, rp, h);
+      } else {
+        // order key: the entry distance of the first lane that enters the child
+        const uint64_t m = __ballot(mine);
+        const int lead = __builtin_amdgcn_readfirstlane(__ffsll((long long)m) - 1);
+        const uint32_t key = (uint32_t)__builtin_amdgcn_readlane((int)__float_as_uint(te[s]), lead);
+        sorted_insert(ik, ic, ni, key, c);
+        ++ni;
+      }
     }
   }
 }
@@ -346,8 +371,8 @@ __device__ __forceinline__ void closest_hit_packet_impl(const SceneView& S, cons
   for (;;) {
     node = __builtin_amdgcn_readfirstlane(node);
     int ni = 0;
-    int ic[4];
-    uint32_t ik[4];
+    int ic[kBvhWidth];
+    uint32_t ik[kBvhWidth];
     ++steps;
     packet_visit<kParallel, kScalar>(S, rp, active, node, h, ic, ik, ni, tris);
     if (ni == 0) {
@@ -356,21 +381,14 @@ __device__ __forceinline__ void closest_hit_packet_impl(const SceneView& S, cons
       node = stack_top(stk0, stk1, sp);
       continue;
     }
-    for (int a = 1; a < ni; ++a) {
-      const uint32_t kv = ik[a];
-      const int cv = ic[a];
-      int b = a - 1;
-      while (b >= 0 && ik[b] > kv) {
-        ik[b + 1] = ik[b];
-        ic[b + 1] = ic[b];
-        --b;
+    // entered children are in near-to-far order: push all but the nearest,
+    // farthest first, and visit the nearest
+#pragma unroll
+    for (int a = kBvhWidth - 1; a >= 1; --a) {
+      if (a < ni) {
+        stack_push(stk0, stk1, lane, sp, ic[a]);
+        ++sp;
       }
-      ik[b + 1] = kv;
-      ic[b + 1] = cv;
-    }
-    for (int a = ni - 1; a >= 1; --a) {
-      stack_push(stk0, stk1, lane, sp, ic[a]);
-      ++sp;
     }
     node = ic[0];
   }
@@ -398,8 +416,34 @@ __device__ __forceinline__ void closest_hit_packet(const SceneView& S, const Ray
     closest_hit_packet_impl<false, kScalar>(S, rp, active, h, steps, tris);
 }
 
+// The value of lane (lane ^ M), M a power of two below 64, without an
+// address register: ds_swizzle in bit mode (and 0x1f, xor M) within each
+// 32-lane half for M <= 16, and gfx950's v_permlane32_swap across the halves
+// for M = 32.  __shfl_xor needs a (lane ^ M) * 4 address per mask, which the
+// compiler computes once per kernel and keeps live (in the unit renderer, six
+// of them spilled to scratch and reloaded in every round).
+template <int M>
+__device__ __forceinline__ uint32_t lane_xor(uint32_t v) {
+  static_assert(M >= 1 && M <= 32 && (M & (M - 1)) == 0, "power of two below 64");
+  if constexpr (M == 32) {
+    const auto r = __builtin_amdgcn_permlane32_swap(v, v, false, false);
+    return (threadIdx.x & 32u) ? r[0] : r[1];
+  } else {
+    return (uint32_t)__builtin_amdgcn_ds_swizzle((int)v, (M << 10) | 0x1f);
+  }
+}
+template <int M>
+__device__ __forceinline__ float lane_xor(float v) {
+  return __uint_as_float(lane_xor<M>(__float_as_uint(v)));
+}
+
 __device__ __forceinline__ float wave_sum(float v) {
-  for (int m = 32; m >= 1; m >>= 1) v = v + __shfl_xor(v, m);
+  v = v + lane_xor<32>(v);
+  v = v + lane_xor<16>(v);
+  v = v + lane_xor<8>(v);
+  v = v + lane_xor<4>(v);
+  v = v + lane_xor<2>(v);
+  v = v + lane_xor<1>(v);
   return v;
 }
 
@@ -414,15 +458,33 @@ __device__ __forceinline__ uint32_t unit_count(const UnitLaunch& U, uint64_t u, 
 // whose samples occupy lanes a .. a + n - 1 (n <= seg, other lanes 0) this is
 // bit-identical to wave_sum over a wave holding the same samples in lanes
 // 0 .. n - 1: the butterfly steps with offset >= seg only add zeros there.
+template <int M>
+__device__ __forceinline__ float segment_step(float v, int seg) {
+  const float w = lane_xor<M>(v);
+  return M < seg ? v + w : v;
+}
 __device__ __forceinline__ float segment_sum(float v, int seg) {
-  for (int m = 32; m >= 1; m >>= 1) {
-    const float w = __shfl_xor(v, m);
-    if (m < seg) v = v + w;
-  }
+  v = segment_step<32>(v, seg);
+  v = segment_step<16>(v, seg);
+  v = segment_step<8>(v, seg);
+  v = segment_step<4>(v, seg);
+  v = segment_step<2>(v, seg);
+  v = segment_step<1>(v, seg);
   return v;
 }
 
 // Pose, ommatidium and output slot of queue position u.
+// The unit renderer's launches hold fewer than 2^32 units (launch_render_units
+// checks), so its queue positions and output slots are 32-bit.
+__device__ __forceinline__ void locate_unit32(const UnitLaunch& U, uint32_t u, int* p, int* i, uint32_t* out) {
+  const uint32_t n = (uint32_t)U.omm_count;
+  const uint32_t q = u / n;
+  const uint32_t k = u - q * n;
+  *p = (int)q;
+  *i = U.order ? (int)U.order[k] : (int)k;
+  *out = q * n + (uint32_t)*i;
+}
+
 __device__ __forceinline__ void locate_unit(const UnitLaunch& U, uint64_t u, int* p, int* i, uint64_t* out) {
   *p = (int)(u / (uint64_t)U.omm_count);
   const int k = (int)(u - (uint64_t)*p * (uint64_t)U.omm_count);
@@ -431,14 +493,26 @@ __device__ __forceinline__ void locate_unit(const UnitLaunch& U, uint64_t u, int
 }
 
 // Bitonic sort of one value per lane, largest first, across the 64 lanes.
+template <int K, int J>
+__device__ __forceinline__ uint32_t sort_step(uint32_t v, int lane) {
+  const uint32_t o = lane_xor<J>(v);
+  const bool take_max = ((lane & J) == 0) == ((lane & K) == 0);
+  return take_max ? (o > v ? o : v) : (o < v ? o : v);
+}
+template <int K, int J>
+__device__ __forceinline__ uint32_t sort_merge(uint32_t v, int lane) {
+  v = sort_step<K, J>(v, lane);
+  if constexpr (J > 1) v = sort_merge<K, J / 2>(v, lane);
+  return v;
+}
 __device__ __forceinline__ uint32_t wave_sort_desc(uint32_t v) {
   const int lane = (int)(threadIdx.x & 63);
-  for (int k = 2; k <= 64; k <<= 1)
-    for (int j = k >> 1; j > 0; j >>= 1) {
-      const uint32_t o = (uint32_t)__shfl_xor((int)v, j);
-      const bool take_max = ((lane & j) == 0) == ((lane & k) == 0);
-      v = take_max ? (o > v ? o : v) : (o < v ? o : v);
-    }
+  v = sort_merge<2, 1>(v, lane);
+  v = sort_merge<4, 2>(v, lane);
+  v = sort_merge<8, 4>(v, lane);
+  v = sort_merge<16, 8>(v, lane);
+  v = sort_merge<32, 16>(v, lane);
+  v = sort_merge<64, 32>(v, lane);
   return v;
 }
 
@@ -505,8 +579,8 @@ __global__ __launch_bounds__(kTraceThreads, kTrav == kUnitPerRay ? 4 : 6) void k
     uint32_t my_n = 0;
     if (lane < G) {
       int p, i;
-      uint64_t out;
-      locate_unit(U, (uint64_t)base + (uint64_t)lane, &p, &i, &out);
+      uint32_t out;
+      locate_unit32(U, (uint32_t)base + (uint32_t)lane, &p, &i, &out);
       my_n = unit_count(U, out, i);
     }
     // Rounds take the grab's units largest first: power-of-two segments in
@@ -546,17 +620,25 @@ __global__ __launch_bounds__(kTraceThreads, kTrav == kUnitPerRay ? 4 : 6) void k
       const int src = slot < 0 ? 0 : slot;
       const int unit_off = __shfl(order, src);
       int p, i;
-      uint64_t out;
-      locate_unit(U, (uint64_t)base + (uint64_t)unit_off, &p, &i, &out);
+      {
+        uint32_t out_unused;
+        locate_unit32(U, (uint32_t)base + (uint32_t)unit_off, &p, &i, &out_unused);
+      }
       const uint32_t n_src = (uint32_t)__shfl((int)my_n, src);  // every lane takes part in the shuffle
       const uint32_t n = slot < 0 ? 0u : n_src;
       const float inv = 1.0f / (float)(n > 0u ? n : 1u);
-      const uint32_t s_lane = (uint32_t)(lane - seg_off);
+      // The lane's place in the round, the one register of round bookkeeping
+      // kept across the traversal (the rest is recomputed after it, which
+      // keeps the six-wave kernel's unit state out of scratch): bit 16 set
+      // when the lane serves a unit, bits 8-10 log2 of its segment, bits 0-6
+      // the lane's first sample in its unit (lane - segment start).
+      const uint32_t desc = (slot >= 0 ? 0x10000u : 0u) | ((uint32_t)__builtin_ctz((unsigned)seg) << 8) |
+                            (uint32_t)(lane - seg_off);
       float r = 0.0f, gg = 0.0f, b = 0.0f;
       float sr = 0.0f, sg = 0.0f, sb = 0.0f, sd = 0.0f, sd2 = 0.0f, sh = 0.0f;
-      for (uint32_t c0 = 0; __ballot(slot >= 0 && c0 + s_lane < n) != 0ull; c0 += 64u) {
-        const uint32_t s = c0 + s_lane;
-        const bool active = slot >= 0 && s < n;
+      for (uint32_t c0 = 0; __ballot(c0 + (desc & 127u) < n) != 0ull; c0 += 64u) {
+        const uint32_t s = c0 + (desc & 127u);
+        const bool active = s < n;
         Ray ray;
         if (active) {
           const U4 rnd = sample_randoms(U.seed, U.eye, (uint32_t)i, U.sample_base + s, U.frame,
@@ -585,7 +667,9 @@ __global__ __launch_bounds__(kTraceThreads, kTrav == kUnitPerRay ? 4 : 6) void k
           gg = gg + c.y * inv;
           b = b + c.z * inv;
           const bool hit = h.id != 0xffffffffu;
-          if (kSamples) U.samples[U.sample_offsets[out] + s] = make_float4(c.x, c.y, c.z, hit ? h.t : -1.0f);
+          if (kSamples)
+            U.samples[U.sample_offsets[(uint32_t)p * (uint32_t)U.omm_count + (uint32_t)i] + s] =
+                make_float4(c.x, c.y, c.z, hit ? h.t : -1.0f);
           if (kStats) {
             sr += c.x * c.x;
             sg += c.y * c.y;
@@ -597,18 +681,20 @@ __global__ __launch_bounds__(kTraceThreads, kTrav == kUnitPerRay ? 4 : 6) void k
           }
         }
       }
-      r = segment_sum(r, seg);
-      gg = segment_sum(gg, seg);
-      b = segment_sum(b, seg);
+      const int seg_len = 1 << ((desc >> 8) & 7u);
+      r = segment_sum(r, seg_len);
+      gg = segment_sum(gg, seg_len);
+      b = segment_sum(b, seg_len);
       if (kStats) {
-        sr = segment_sum(sr, seg);
-        sg = segment_sum(sg, seg);
-        sb = segment_sum(sb, seg);
-        sd = segment_sum(sd, seg);
-        sd2 = segment_sum(sd2, seg);
-        sh = segment_sum(sh, seg);
+        sr = segment_sum(sr, seg_len);
+        sg = segment_sum(sg, seg_len);
+        sb = segment_sum(sb, seg_len);
+        sd = segment_sum(sd, seg_len);
+        sd2 = segment_sum(sd2, seg_len);
+        sh = segment_sum(sh, seg_len);
       }
-      if (slot >= 0 && lane == seg_off) {
+      if ((desc & 0x1007fu) == 0x10000u) {   // the first lane of a unit's segment
+        const uint32_t out = (uint32_t)p * (uint32_t)U.omm_count + (uint32_t)i;
         U.colours[out] = make_float4(r, gg, b, (float)n);
         if (kStats) {
           // E[c^2] - E[c]^2 with E[c] the unit's mean colour
@@ -616,8 +702,8 @@ __global__ __launch_bounds__(kTraceThreads, kTrav == kUnitPerRay ? 4 : 6) void k
           const float vg = fmaxf(sg * inv - gg * gg, 0.0f);
           const float vb = fmaxf(sb * inv - b * b, 0.0f);
           const float md = sd * inv;
-          U.stats[2 * out] = make_float4(vr, vg, vb, sh * inv);
-          U.stats[2 * out + 1] = make_float4(md, fmaxf(sd2 * inv - md * md, 0.0f), 0.0f, 0.0f);
+          U.stats[2 * (size_t)out] = make_float4(vr, vg, vb, sh * inv);
+          U.stats[2 * (size_t)out + 1] = make_float4(md, fmaxf(sd2 * inv - md * md, 0.0f), 0.0f, 0.0f);
         }
       }
       g = g_end;
@@ -625,133 +711,6 @@ __global__ __launch_bounds__(kTraceThreads, kTrav == kUnitPerRay ? 4 : 6) void k
   }
 }
 
-// Split launch of units with a uniform sample count n > 64: task t is chunk
-// t % C of unit t / C (C = ceil(n / 64)), so that a single pose's few units
-// still fill the GPU.  Lane l of a task keeps the terms of its one sample
-// (scaled colour, and the statistics terms) in `partial`; the wave that
-// completes a unit's last chunk adds the chunks lane by lane in chunk order and
-// then across the wave, which gives the same sums as k_render_units (there a
-// lane adds samples l, l + 64, ... in this order).
-//
-// The rays of a task are traversed one per lane (an LDS stack column per
-// lane).  The wave-packet traversal is not used here: in this kernel it did
-// not terminate on the GPU, and its wave stack is written by lane 0 alone.
-template <bool kStats, bool kSamples>
-__global__ __launch_bounds__(kTraceThreads, 5) void k_render_chunks(SceneView S, UnitLaunch U, int chunks) {
-  extern __shared__ uint32_t lane_stacks[];
-  const int lane = threadIdx.x & 63;
-  uint32_t* lane_stack = lane_stacks + threadIdx.x;
-  constexpr int kPer = kStats ? 3 : 1;
-  const uint64_t units = (uint64_t)U.pose_count * (uint64_t)U.omm_count;
-  const uint64_t tasks = units * (uint64_t)chunks;
-  const uint32_t n = (uint32_t)U.spp;
-  const float inv = 1.0f / (float)n;
-  for (;;) {
-    uint32_t t = 0;
-    if (lane == 0) t = atomicAdd(U.queue, 1u);
-    t = __builtin_amdgcn_readfirstlane(__shfl(t, 0));
-    if ((uint64_t)t >= tasks) break;
-    const uint64_t uq = (uint64_t)t / (uint64_t)chunks;
-    const uint32_t j = t - (uint32_t)(uq * (uint64_t)chunks);
-    int p, i;
-    uint64_t out;
-    locate_unit(U, uq, &p, &i, &out);
-    const uint32_t s = j * 64u + (uint32_t)lane;
-    const bool active = s < n;
-    Ray ray;
-    if (active) {
-      const U4 rnd = sample_randoms(U.seed, U.eye, (uint32_t)i, U.sample_base + s, U.frame,
-                                    U.pose_base + (uint32_t)p);
-      ray = generate_ray(U.omms[i], U.poses[p], rnd);
-    } else {
-      ray.ox = ray.oy = ray.oz = 0.0f;
-      ray.dx = ray.dy = 0.0f;
-      ray.dz = 1.0f;
-      ray.tmin = 0.0f;
-      ray.tmax = 1.0f;
-    }
-    const RayPre rp = precompute_ray(ray);
-    Hit h;
-    h.t = rp.tmax;
-    h.k = h.id = 0xffffffffu;
-    h.b1 = h.b2 = 0.0f;
-    if (active) closest_hit(S, rp, lane_stack, (int)blockDim.x, h);
-    float4 a = make_float4(0.0f, 0.0f, 0.0f, 0.0f), b = a, c2 = a;
-    if (active) {
-      const V3 c = shade(S, h, ray.dx, ray.dy, ray.dz);
-      const bool hit = h.id != 0xffffffffu;
-      a = make_float4(c.x * inv, c.y * inv, c.z * inv, 0.0f);
-      if (kSamples) U.samples[U.sample_offsets[out] + s] = make_float4(c.x, c.y, c.z, hit ? h.t : -1.0f);
-      if (kStats) {
-        const float idp = hit ? 1.0f / (1.0f + h.t) : 0.0f;
-        b = make_float4(c.x * c.x, c.y * c.y, c.z * c.z, idp);
-        c2 = make_float4(idp * idp, hit ? 1.0f : 0.0f, 0.0f, 0.0f);
-      }
-    }
-    float4* slot = U.chunk_partial + ((uq * (uint64_t)chunks + j) * 64u + (uint64_t)lane) * kPer;
-    slot[0] = a;
-    if (kStats) {
-      slot[1] = b;
-      slot[2] = c2;
-    }
-    __threadfence();
-    uint32_t prev = 0;
-    if (lane == 0) prev = atomicAdd(U.chunk_done + uq, 1u);
-    prev = __builtin_amdgcn_readfirstlane(__shfl(prev, 0));
-    if (prev != (uint32_t)chunks - 1u) continue;
-    __threadfence();
-    float r = 0.0f, gg = 0.0f, bb = 0.0f;
-    float sr = 0.0f, sg = 0.0f, sb = 0.0f, sd = 0.0f, sd2 = 0.0f, sh = 0.0f;
-    for (int q = 0; q < chunks; ++q) {
-      const float4* src = U.chunk_partial + ((uq * (uint64_t)chunks + (uint64_t)q) * 64u + (uint64_t)lane) * kPer;
-      if (q * 64 + lane >= (int)n) break;
-      const float4 v = src[0];
-      r = r + v.x;
-      gg = gg + v.y;
-      bb = bb + v.z;
-      if (kStats) {
-        const float4 w = src[1], x = src[2];
-        sr += w.x;
-        sg += w.y;
-        sb += w.z;
-        sd += w.w;
-        sd2 += x.x;
-        sh += x.y;
-      }
-    }
-    r = segment_sum(r, 64);
-    gg = segment_sum(gg, 64);
-    bb = segment_sum(bb, 64);
-    if (kStats) {
-      sr = segment_sum(sr, 64);
-      sg = segment_sum(sg, 64);
-      sb = segment_sum(sb, 64);
-      sd = segment_sum(sd, 64);
-      sd2 = segment_sum(sd2, 64);
-      sh = segment_sum(sh, 64);
-    }
-    if (lane == 0) {
-      U.colours[out] = make_float4(r, gg, bb, (float)n);
-      if (kStats) {
-        const float vr = fmaxf(sr * inv - r * r, 0.0f);
-        const float vg = fmaxf(sg * inv - gg * gg, 0.0f);
-        const float vb = fmaxf(sb * inv - bb * bb, 0.0f);
-        const float md = sd * inv;
-        U.stats[2 * out] = make_float4(vr, vg, vb, sh * inv);
-        U.stats[2 * out + 1] = make_float4(md, fmaxf(sd2 * inv - md * md, 0.0f), 0.0f, 0.0f);
-      }
-    }
-  }
-}
-
-// Non-persistent form of the split launch for wave-packet traversal: wave w
-// of logical block L is task 4L + w (chunk of queue position task / C), the
-// whole wave returns together when past the last task, and nothing but the
-// traversal sits between the ray generation and the chunk store, which keeps
-// register use low (more resident waves).  Same per-lane chunk values,
-// completion counters and last-wave reduction as k_render_chunks (no
-// statistics).
-//
 // Workgroups are handed to the 8 XCDs round-robin (block b to XCD b % 8),
 // and each XCD has its own L2.  Logical blocks are therefore dealt to XCDs in
 // runs of kXcdRun consecutive blocks: the queue is in Morton order of the
@@ -767,6 +726,125 @@ __device__ __forceinline__ uint32_t xcd_logical_block(uint32_t b, uint32_t block
   const uint32_t x = b % kXcds, j = b / kXcds;
   return ((j / kXcdRun) * kXcds + x) * kXcdRun + j % kXcdRun;
 }
+
+// Split launch of units with a uniform sample count n, per-ray traversal:
+// task t is chunk t % C of queue position t / C (C = ceil(n / 64)), so that a
+// single pose's few units still fill the GPU.  Wave w of logical block L is
+// task 4L + w; there is no task loop.  Lane l of a task keeps the terms of
+// its one sample (scaled colour, and the statistics terms) in `partial`; the
+// wave that completes a unit's last chunk adds the chunks lane by lane in
+// chunk order and then across the wave, which gives the same sums as
+// k_render_units (there a lane adds samples l, l + 64, ... in this order).
+//
+// The rays of a task are traversed one per lane (an LDS stack column per
+// lane), so lanes past n simply skip the traversal.
+//
+// Why no persistent task loop: a loop whose body takes its task from a lane-0
+// atomic broadcast with __shfl and ends in lane-0-only stores is compiled
+// with the lane-0 branch leaving the loop on its own; the other 63 lanes
+// then go round without lane 0, read 0 from the broadcast (a permute from a
+// disabled lane) and, for C = 1, never leave.  Every wave here runs
+// straight through, and its lane-dependent branches rejoin before the next
+// cross-lane operation.
+template <bool kStats, bool kSamples>
+__global__ __launch_bounds__(kTraceThreads, 5) void k_render_chunks(SceneView S, UnitLaunch U, int chunks) {
+  extern __shared__ uint32_t lane_stacks[];
+  const int lane = threadIdx.x & 63;
+  uint32_t* lane_stack = lane_stacks + threadIdx.x;
+  constexpr int kPer = kStats ? 3 : 1;
+  const uint64_t units = (uint64_t)U.pose_count * (uint64_t)U.omm_count;
+  const uint64_t tasks = units * (uint64_t)chunks;
+  const uint64_t t = (uint64_t)xcd_logical_block(blockIdx.x, gridDim.x) * (kTraceThreads / 64) +
+                     (uint64_t)(threadIdx.x >> 6);
+  if (t >= tasks) return;  // whole wave
+  const uint32_t n = (uint32_t)U.spp;
+  const float inv = 1.0f / (float)n;
+  const uint64_t uq = t / (uint64_t)chunks;
+  const uint32_t j = (uint32_t)(t - uq * (uint64_t)chunks);
+  int p, i;
+  uint64_t out;
+  locate_unit(U, uq, &p, &i, &out);
+  const uint32_t s = j * 64u + (uint32_t)lane;
+  const bool active = s < n;
+  float4 a = make_float4(0.0f, 0.0f, 0.0f, 0.0f), b = a, c2 = a;
+  if (active) {
+    const U4 rnd = sample_randoms(U.seed, U.eye, (uint32_t)i, U.sample_base + s, U.frame,
+                                  U.pose_base + (uint32_t)p);
+    const Ray ray = generate_ray(U.omms[i], U.poses[p], rnd);
+    const RayPre rp = precompute_ray(ray);
+    Hit h;
+    closest_hit(S, rp, lane_stack, (int)blockDim.x, h);
+    const V3 c = shade(S, h, ray.dx, ray.dy, ray.dz);
+    const bool hit = h.id != 0xffffffffu;
+    a = make_float4(c.x * inv, c.y * inv, c.z * inv, 0.0f);
+    if (kSamples) U.samples[U.sample_offsets[out] + s] = make_float4(c.x, c.y, c.z, hit ? h.t : -1.0f);
+    if (kStats) {
+      const float idp = hit ? 1.0f / (1.0f + h.t) : 0.0f;
+      b = make_float4(c.x * c.x, c.y * c.y, c.z * c.z, idp);
+      c2 = make_float4(idp * idp, hit ? 1.0f : 0.0f, 0.0f, 0.0f);
+    }
+  }
+  float4* slot = U.chunk_partial + ((uq * (uint64_t)chunks + j) * 64u + (uint64_t)lane) * kPer;
+  slot[0] = a;
+  if (kStats) {
+    slot[1] = b;
+    slot[2] = c2;
+  }
+  __threadfence();
+  uint32_t prev = 0;
+  if (lane == 0) prev = atomicAdd(U.chunk_done + uq, 1u);
+  prev = __builtin_amdgcn_readfirstlane(__shfl(prev, 0));
+  if (prev != (uint32_t)chunks - 1u) return;  // whole wave
+  __threadfence();
+  float r = 0.0f, gg = 0.0f, bb = 0.0f;
+  float sr = 0.0f, sg = 0.0f, sb = 0.0f, sd = 0.0f, sd2 = 0.0f, sh = 0.0f;
+  for (int q = 0; q < chunks; ++q) {
+    if (q * 64 + lane < (int)n) {
+      const float4* src = U.chunk_partial + ((uq * (uint64_t)chunks + (uint64_t)q) * 64u + (uint64_t)lane) * kPer;
+      const float4 v = src[0];
+      r = r + v.x;
+      gg = gg + v.y;
+      bb = bb + v.z;
+      if (kStats) {
+        const float4 w = src[1], x = src[2];
+        sr += w.x;
+        sg += w.y;
+        sb += w.z;
+        sd += w.w;
+        sd2 += x.x;
+        sh += x.y;
+      }
+    }
+  }
+  r = segment_sum(r, 64);
+  gg = segment_sum(gg, 64);
+  bb = segment_sum(bb, 64);
+  if (kStats) {
+    sr = segment_sum(sr, 64);
+    sg = segment_sum(sg, 64);
+    sb = segment_sum(sb, 64);
+    sd = segment_sum(sd, 64);
+    sd2 = segment_sum(sd2, 64);
+    sh = segment_sum(sh, 64);
+  }
+  if (lane == 0) {
+    U.colours[out] = make_float4(r, gg, bb, (float)n);
+    if (kStats) {
+      const float vr = fmaxf(sr * inv - r * r, 0.0f);
+      const float vg = fmaxf(sg * inv - gg * gg, 0.0f);
+      const float vb = fmaxf(sb * inv - bb * bb, 0.0f);
+      const float md = sd * inv;
+      U.stats[2 * out] = make_float4(vr, vg, vb, sh * inv);
+      U.stats[2 * out + 1] = make_float4(md, fmaxf(sd2 * inv - md * md, 0.0f), 0.0f, 0.0f);
+    }
+  }
+}
+
+// The same split launch for wave-packet traversal: the whole wave returns
+// together when past the last task, and nothing but the traversal sits
+// between the ray generation and the chunk store, which keeps register use
+// low (more resident waves).  Same per-lane chunk values, completion
+// counters and last-wave reduction as k_render_chunks (no statistics).
 template <bool kSamples, bool kScalar>
 __global__ __launch_bounds__(kTraceThreads, 6) void k_render_tasks(SceneView S, UnitLaunch U, int chunks) {
   const int lane = threadIdx.x & 63;
@@ -820,11 +898,12 @@ __global__ __launch_bounds__(kTraceThreads, 6) void k_render_tasks(SceneView S, 
   __threadfence();
   float r = 0.0f, gg = 0.0f, bb = 0.0f;
   for (int q = 0; q < chunks; ++q) {
-    if (q * 64 + lane >= (int)n) break;
-    const float4 v = U.chunk_partial[(uq * (uint64_t)chunks + (uint64_t)q) * 64u + (uint64_t)lane];
-    r = r + v.x;
-    gg = gg + v.y;
-    bb = bb + v.z;
+    if (q * 64 + lane < (int)n) {
+      const float4 v = U.chunk_partial[(uq * (uint64_t)chunks + (uint64_t)q) * 64u + (uint64_t)lane];
+      r = r + v.x;
+      gg = gg + v.y;
+      bb = bb + v.z;
+    }
   }
   r = segment_sum(r, 64);
   gg = segment_sum(gg, 64);
@@ -1083,16 +1162,26 @@ bool launch_render_units(const DeviceScene& scene, const UnitLaunch& U_in, hipSt
   const int capacity = render_units_capacity(scene, U);
   int blocks = U.blocks > 0 ? U.blocks : capacity;
   const SceneView v = view_of(scene);
-  // Split launch: uniform counts above 64 samples and fewer units than two per
-  // resident wave (a single pose, typically), when the caller gave scratch.
+  // Split launch: uniform counts and fewer units than two per wave slot of
+  // the device (a single pose, typically), when the caller gave scratch: each
+  // unit's 64-sample chunks are tasks of their own, one wave each.  With
+  // per-ray traversal or statistics, units of 32 samples or fewer stay in the
+  // unit kernel, which packs two or more of them into a wave's lanes (a
+  // per-ray task would leave the other lanes idle); the packet task kernel
+  // takes any count (a single pose still renders sooner that way).  The slots
+  // are counted at the kernels' launch bound (5 blocks of 4 waves per CU) or
+  // the unit kernel's occupancy if higher, not lowered by the LDS the per-ray
+  // stack takes: a deep tree's per-ray stack cuts the persistent unit kernel
+  // to two blocks per CU, which leaves a single pose badly balanced.
   const uint64_t chunks = U.spp > 64 ? ((uint64_t)U.spp + 63u) / 64u : 1u;
-  const uint64_t resident_waves = (uint64_t)capacity * (kTraceThreads / 64);
+  const uint64_t resident_waves = (uint64_t)std::max(capacity, device_cus() * 5) * (kTraceThreads / 64);
   const uint64_t per_chunk = U.stats ? 3u : 1u;
-  if (g_split && !U.omm_counts && !U.unit_counts && chunks > 1 && units < 2 * resident_waves && U.chunk_partial &&
-      U.chunk_done && units * chunks <= 0xffffffffull - 65536ull &&
+  const bool packet_tasks = U.traversal != kUnitPerRay && !U.stats;
+  if (g_split && !U.omm_counts && !U.unit_counts && (U.spp > 32 || packet_tasks) && units < 2 * resident_waves &&
+      U.chunk_partial && U.chunk_done && units * chunks <= 0xffffffffull - 65536ull &&
       units * chunks * 64u * per_chunk <= U.chunk_partial_cap && units <= U.chunk_done_cap) {
     const uint64_t tasks = units * chunks;
-    if (U.traversal != kUnitPerRay && !U.stats && tasks / (kTraceThreads / 64) < 0x7fffffffull) {
+    if (packet_tasks && tasks / (kTraceThreads / 64) < 0x7fffffffull) {
       // wave packets: the non-persistent task kernel, one wave per chunk
       hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((units + 255) / 256)), dim3(256), 0, stream, U.chunk_done,
                          (uint32_t)units, 0u);
@@ -1109,25 +1198,18 @@ bool launch_render_units(const DeviceScene& scene, const UnitLaunch& U_in, hipSt
       g_last_kernel = "k_render_tasks";
       return check_launch("render_tasks", err);
     }
-    // per-ray traversal: the persistent chunk kernel
+    // per-ray traversal: the chunk kernel, one wave per chunk, an LDS stack
+    // column per lane
     const size_t chunk_lds = stack_bytes(scene);
     if (chunk_lds > 160 * 1024) {
       if (err) *err = "render_chunks: BVH too deep for the LDS traversal stack";
       return false;
     }
-    int chunk_blocks = U.blocks;
-    if (chunk_blocks <= 0) {
-      int per_cu = 0;
-      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
-              &per_cu, reinterpret_cast<const void*>(&k_render_chunks<false, false>), kTraceThreads, chunk_lds) !=
-              hipSuccess ||
-          per_cu <= 0)
-        per_cu = 4;
-      chunk_blocks = device_cus() * per_cu;
+    if (tasks / (kTraceThreads / 64) >= 0x7fffffffull) {
+      if (err) *err = "render_chunks: too many chunks for one launch";
+      return false;
     }
-    const uint64_t waves = (uint64_t)chunk_blocks * (kTraceThreads / 64);
-    if (waves > tasks) chunk_blocks = (int)((tasks + (kTraceThreads / 64) - 1) / (kTraceThreads / 64));
-    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, stream, U.queue, (uint32_t)kQueueWords, 0u);
+    const int chunk_blocks = (int)((tasks + (kTraceThreads / 64) - 1) / (kTraceThreads / 64));
     hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((units + 255) / 256)), dim3(256), 0, stream, U.chunk_done,
                        (uint32_t)units, 0u);
     launch_chunks(U, v, chunk_blocks, chunk_lds, (int)chunks, stream);

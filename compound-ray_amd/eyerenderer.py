@@ -58,6 +58,8 @@ class Ommatidium:
         return Ommatidium(self.position.copy(), self.direction.copy(), self.acceptanceAngle, self.focalpointOffset)
 
 
+
+
 def configure_functions(lib):
     """Argument/result types of the reference ABI (libEyeRenderer.h:16-67)."""
     lib.setVerbosity.argtypes = [c_bool]
@@ -449,12 +451,12 @@ def bvh_info(lib) -> dict:
     if lib.getSceneBVHInfo(info) != 0:
         raise RuntimeError("no scene loaded")
     return {"tris": int(info[0]), "nodes": int(info[1]), "stack_bound": int(info[2]), "levels": int(info[3]),
-            "leaf_size": int(info[4]), "build_ms": info[5] / 1000.0}
+            "leaf_size": int(info[4]), "build_ms": info[5] / 1000.0, "node_bytes": int(info[6])}
 
 
 def bvh_nodes_np(lib) -> np.ndarray:
     info = bvh_info(lib)
-    buf = np.zeros(info["nodes"] * 128, dtype=np.uint8)
+    buf = np.zeros(info["nodes"] * info["node_bytes"], dtype=np.uint8)
     got = lib.copySceneBVHNodes(buf.ctypes.data_as(c_void_p), buf.size)
     return buf[:got]
 

@@ -257,7 +257,8 @@ double renderPoseBatchAdaptive(const float* poses, size_t poseCount, float error
                                float budgetScale, float* rgbOut, uint32_t* countsOut, uint64_t* totalSamples);
 
 /* Acceleration-structure inspection: info[0..7] = triangle count, node count,
- * stack bound, levels, leaf size, build ms * 1000, 0, 0.  Returns 0 or -1. */
+ * stack bound, levels, leaf size, build ms * 1000, bytes per node (128 per
+ * four child slots), 0.  Returns 0 or -1. */
 int getSceneBVHInfo(int64_t* info);
 /* Copies the 4-wide node array (128 bytes per node); returns bytes copied. */
 size_t copySceneBVHNodes(void* dst, size_t bytes);

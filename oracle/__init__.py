@@ -61,6 +61,8 @@ def lib():
     return _lib
 
 
+
+
 def _p(a):
     return None if a is None else a.ctypes.data_as(c_void_p)
 
@@ -142,16 +144,16 @@ class OracleScene:
             self.h = None
 
     def info(self) -> dict:
-        info = np.zeros(11, dtype=np.int64)
+        info = np.zeros(12, dtype=np.int64)
         lib().oracle_scene_info(self.h, _p(info))
         return {"tris": int(info[0]), "nodes": int(info[1]), "stack_bound": int(info[2]), "levels": int(info[3]),
-                "leaf_size": int(info[4]),
+                "leaf_size": int(info[4]), "node_bytes": int(info[11]),
                 "centroid_lo": info[5:8].astype(np.uint32).view(np.float32),
                 "centroid_hi": info[8:11].astype(np.uint32).view(np.float32)}
 
     def nodes(self) -> np.ndarray:
-        n = self.info()["nodes"]
-        out = np.zeros(n * 128, dtype=np.uint8)
+        info = self.info()
+        out = np.zeros(info["nodes"] * info["node_bytes"], dtype=np.uint8)
         lib().oracle_scene_nodes(self.h, _p(out))
         return out
 

@@ -27,7 +27,7 @@
 //   acceleration     LBVH with the deterministic rules of the GPU builder
 //                    (Morton codes from FP64-quantised centroids, stable
 //                    (code, id) sort, Karras 2012 hierarchy, min/max refit,
-//                    breadth-first 4-wide collapse by largest surface area)
+//                    breadth-first collapse to WIDTH children by largest surface area)
 //   learned sampler  FP64 MLP forward/backward for gradient checks
 // Transcendentals use the same published series as the GPU library (Taylor,
 // atanh-series log, Maclaurin asin) built from IEEE +,-,*,/,fma,sqrt only, so
@@ -293,12 +293,24 @@ bool hit_triangle(const Pre& r, const float* v, float* t_out, float* b1_out, flo
 constexpr int32_t EMPTY = 0x7fffffff;
 constexpr float EMPTY_BOX = 3.0e38f;
 
-struct Node4 {
+// Nodes have up to WIDTH children (4 or 8, the library's build width).
+// Memory layout (32 * WIDTH bytes): WIDTH / 4 halves of 128 bytes with four
+// slots each, every half holding its slots' boxes as arrays (lo x, hi x,
+// lo y, hi y, lo z, hi z), the child words and four padding words; slot s is
+// in half s / 4 at position s % 4.
+constexpr int WIDTH = 4;
+struct NodeHalf {
   float lox[4], hix[4], loy[4], hiy[4], loz[4], hiz[4];
   int32_t child[4];
   uint32_t pad[4];
 };
-static_assert(sizeof(Node4) == 128, "node layout");
+struct Node {
+  NodeHalf h[WIDTH / 4];
+  const NodeHalf& of(int s) const { return h[s / 4]; }
+  NodeHalf& of(int s) { return h[s / 4]; }
+  int32_t child(int s) const { return h[s / 4].child[s % 4]; }
+};
+static_assert(sizeof(Node) == 32 * WIDTH, "node layout");
 
 float mn(float a, float b) { return a < b ? a : b; }
 float mx(float a, float b) { return a > b ? a : b; }
@@ -337,7 +349,7 @@ struct Scene {
   std::vector<int32_t> tex;
   std::vector<uint32_t> texels;      // RGBA8, red in the low byte
   std::vector<uint32_t> codes, ids;  // sorted
-  std::vector<Node4> nodes;
+  std::vector<Node> nodes;
   std::vector<float> stris;  // 9 per sorted triangle
   int stack_bound = 0;
   int levels = 0;
@@ -455,7 +467,7 @@ void build(Scene& S) {
     const size_t next_base = base + cur.size();
     for (size_t j = 0; j < cur.size(); ++j) {
       const int src = cur[j];
-      int ch[4];
+      int ch[WIDTH];
       int c;
       if (is_leafish(src)) {
         ch[0] = src;
@@ -464,7 +476,7 @@ void build(Scene& S) {
         ch[0] = left[(size_t)src];
         ch[1] = right[(size_t)src];
         c = 2;
-        while (c < 4) {
+        while (c < WIDTH) {
           int best = -1;
           float best_a = -1.0f;
           for (int k = 0; k < c; ++k) {
@@ -482,24 +494,26 @@ void build(Scene& S) {
       }
       int k_int = 0;
       for (int k = 0; k < c; ++k) k_int += is_leafish(ch[k]) ? 0 : 1;
-      Node4 node;
+      Node node;
       std::memset(&node, 0, sizeof(node));
-      for (int s = 0; s < 4; ++s) {
+      for (int s = 0; s < WIDTH; ++s) {
+        NodeHalf& hf = node.of(s);
+        const int e = s % 4;
         if (s < c) {
           const float* q = &bnd[6 * (size_t)ch[s]];
-          node.lox[s] = q[0]; node.loy[s] = q[1]; node.loz[s] = q[2];
-          node.hix[s] = q[3]; node.hiy[s] = q[4]; node.hiz[s] = q[5];
+          hf.lox[e] = q[0]; hf.loy[e] = q[1]; hf.loz[e] = q[2];
+          hf.hix[e] = q[3]; hf.hiy[e] = q[4]; hf.hiz[e] = q[5];
           if (!is_leafish(ch[s])) {
-            node.child[s] = (int32_t)(next_base + nxt.size());
+            hf.child[e] = (int32_t)(next_base + nxt.size());
             nxt.push_back(ch[s]);
             depth_nxt.push_back(depth_cur[j] + (k_int - 1));
           } else {
-            node.child[s] = (int32_t)(0x80000000u | ((uint32_t)(size_of(ch[s]) - 1) << 27) |
-                                      (uint32_t)rfirst[(size_t)ch[s]]);
+            hf.child[e] = (int32_t)(0x80000000u | ((uint32_t)(size_of(ch[s]) - 1) << 27) |
+                                    (uint32_t)rfirst[(size_t)ch[s]]);
           }
         } else {
-          node.lox[s] = node.hix[s] = node.loy[s] = node.hiy[s] = node.loz[s] = node.hiz[s] = EMPTY_BOX;
-          node.child[s] = EMPTY;
+          hf.lox[e] = hf.hix[e] = hf.loy[e] = hf.hiy[e] = hf.loz[e] = hf.hiz[e] = EMPTY_BOX;
+          hf.child[e] = EMPTY;
         }
       }
       S.stack_bound = std::max(S.stack_bound, depth_cur[j] + std::max(k_int - 1, 0));
@@ -546,11 +560,13 @@ void slab_axis(const Pre& r, int a, float lo, float hi, float* t0, float* t1) {
   }
 }
 
-float slab(const Pre& r, const Node4& nd, int s, float limit) {
+float slab(const Pre& r, const Node& nd, int s, float limit) {
+  const NodeHalf& hf = nd.of(s);
+  const int e = s % 4;
   float x0, x1, y0, y1, z0, z1;
-  slab_axis(r, 0, nd.lox[s], nd.hix[s], &x0, &x1);
-  slab_axis(r, 1, nd.loy[s], nd.hiy[s], &y0, &y1);
-  slab_axis(r, 2, nd.loz[s], nd.hiz[s], &z0, &z1);
+  slab_axis(r, 0, hf.lox[e], hf.hix[e], &x0, &x1);
+  slab_axis(r, 1, hf.loy[e], hf.hiy[e], &y0, &y1);
+  slab_axis(r, 2, hf.loz[e], hf.hiz[e], &z0, &z1);
   const float tn = std::fmax(std::fmax(std::fmin(x0, x1), std::fmin(y0, y1)), std::fmax(std::fmin(z0, z1), r.tmin));
   const float tf = std::fmin(std::fmin(std::fmax(x0, x1), std::fmax(y0, y1)), std::fmax(z0, z1)) * 1.00000048f;
   return tn <= std::fmin(tf, limit) ? tn : INFINITY;
@@ -566,15 +582,16 @@ HitO trace_bvh(const Scene& S, const RayO& r, uint64_t* nodes_visited, uint64_t*
   stack.reserve(64);
   int node = 0;
   for (;;) {
-    const Node4& nd = S.nodes[(size_t)node];
+    const Node& nd = S.nodes[(size_t)node];
     ++*nodes_visited;
-    float te[4];
-    // cull only boxes entered beyond the closest hit by more than 2^-16 relative
-    for (int s = 0; s < 4; ++s) te[s] = slab(p, nd, s, h.t * 1.0000152587890625f);
-    int ni = 0, ic[4];
-    float it[4];
-    for (int s = 0; s < 4; ++s) {
-      const int32_t c = nd.child[s];
+    float te[WIDTH];
+    // all entry distances first, with the culling limit of the hit so far:
+    // cull only boxes entered beyond it by more than 2^-16 relative
+    for (int s = 0; s < WIDTH; ++s) te[s] = slab(p, nd, s, h.t * 1.0000152587890625f);
+    int ni = 0, ic[WIDTH];
+    float it[WIDTH];
+    for (int s = 0; s < WIDTH; ++s) {
+      const int32_t c = nd.child(s);
       if (c == EMPTY || !(te[s] < INFINITY)) continue;
       if (c < 0) {
         const uint32_t first = (uint32_t)c & 0x07ffffffu, cnt = (((uint32_t)c >> 27) & 0xfu) + 1u;
@@ -772,7 +789,8 @@ void oracle_texture_sample(void* h, uint32_t t, float u, float v, float* rgb) {
   rgb[2] = c.z;
 }
 
-// info: tris, nodes, stack bound, levels, leaf size, centroid lo xyz/hi xyz as float bits (6)
+// info: tris, nodes, stack bound, levels, leaf size, centroid lo xyz/hi xyz as float bits (6),
+// node bytes
 void oracle_scene_info(void* h, int64_t* info) {
   const Scene& S = *static_cast<Scene*>(h);
   info[0] = S.n;
@@ -780,6 +798,7 @@ void oracle_scene_info(void* h, int64_t* info) {
   info[2] = S.stack_bound;
   info[3] = S.levels;
   info[4] = S.leaf_max;
+  info[11] = (int64_t)sizeof(Node);
   for (int a = 0; a < 3; ++a) {
     info[5 + a] = S.n ? (int64_t)bits(S.clo[a]) : 0;
     info[8 + a] = S.n ? (int64_t)bits(S.chi[a]) : 0;
@@ -788,7 +807,7 @@ void oracle_scene_info(void* h, int64_t* info) {
 
 void oracle_scene_nodes(void* h, void* dst) {
   const Scene& S = *static_cast<Scene*>(h);
-  if (!S.nodes.empty()) std::memcpy(dst, S.nodes.data(), S.nodes.size() * sizeof(Node4));
+  if (!S.nodes.empty()) std::memcpy(dst, S.nodes.data(), S.nodes.size() * sizeof(Node));
 }
 
 void oracle_scene_order(void* h, uint32_t* ids, uint32_t* codes) {

@@ -175,3 +175,17 @@ def test_learned_sampler_gain(lib, config1):
     print(f"uniform errors {uniform}; adaptive error {e_a:.5f} with {per_omm:.1f} spp; "
           f"uniform needs {need:.1f} spp; gain {gain:.2f}x")
     assert gain > 1.2, (uniform, e_a, per_omm)
+    # the network-free allocation (pooled pilot standard deviation of the unit
+    # and its neighbours) on the same pilots, rated the same way
+    assert lib.setAdaptiveSampler(1) == 0
+    try:
+        lib.setCurrentEyeFrameIndex(0)
+        _, rgb_nf, _, total_nf = er.render_adaptive_np(lib, test_poses, error_target=0.01)
+    finally:
+        assert lib.setAdaptiveSampler(0) == 0
+    e_nf = rms(rgb_nf, ref)
+    per_nf = total_nf / (len(test_poses) * n_omm)
+    need_nf = math.exp(np.interp(-math.log(e_nf), -ys, xs))
+    gain_nf = need_nf / per_nf
+    print(f"network-free error {e_nf:.5f} with {per_nf:.1f} spp; uniform needs {need_nf:.1f} spp; "
+          f"gain {gain_nf:.2f}x; learned over network-free {gain / gain_nf:.2f}x")

@@ -127,6 +127,36 @@ def test_traversal_modes_bit_exact(lib, config1, oracle_c1, mode, spp):
         lib.setTraversalMode(1)
 
 
+@pytest.mark.parametrize("mode", [0, 1])
+@pytest.mark.parametrize("spp", [1, 32, 33, 64, 65])
+def test_split_launch_routing(lib, config1, oracle_c1, mode, spp):
+    """One pose with uniform counts: each unit's 64-sample chunks are tasks of
+    their own (k_render_tasks for packets at any count, k_render_chunks per
+    ray above 32 samples); per ray at 32 or fewer the unit kernel packs units
+    into lanes.  Either way, and with split launches switched off, the
+    colours are the oracle's."""
+    lib.setTraversalMode(mode)
+    try:
+        omm = config1["eyes"][0][::2]
+        setup_eye(lib, config1, omm=omm, spp=spp, record=False)
+        for split in (True, False):
+            lib.setSplitLaunches(split)
+            assert lib.renderFrame() >= 0, er.last_error(lib)
+            want = b"k_render_units"
+            if split and mode == 1:
+                want = b"k_render_tasks"
+            elif split and spp > 32:
+                want = b"k_render_chunks"
+            assert lib.getLastRenderKernel() == want
+            ref = oracle_c1.render(omm, er.pose_np(lib), spp=spp, eye=lib.getCurrentEyeIndex(),
+                                   frame=lib.getCurrentEyeFrameIndex() - 1)
+            assert np.array_equal(er.colours_np(lib), ref["colour_gpu_order"][0])
+    finally:
+        lib.setSplitLaunches(True)
+        lib.setTraversalMode(1)
+        lib.setSampleRecording(True)
+
+
 def test_unrecorded_render_matches(lib, config1, oracle_c1):
     """Default renderFrame path (samples reduced in registers, nothing recorded)."""
     setup_eye(lib, config1, spp=48, record=False)

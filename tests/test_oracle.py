@@ -373,9 +373,14 @@ def test_make_color_table():
 
 def test_lbvh_invariants(config1, oracle_c1):
     info = oracle_c1.info()
-    nodes = oracle_c1.nodes().view(np.uint8).reshape(-1, 128)
-    f = nodes[:, :96].copy().view(np.float32).reshape(-1, 6, 4)   # lox hix loy hiy loz hiz x 4 slots
-    child = nodes[:, 96:112].copy().view(np.int32).reshape(-1, 4)
+    # nodes of 128-byte halves: (lox hix loy hiy loz hiz) x 4 slots, 4 child words, 4 pad words
+    nh = info["node_bytes"] // 128
+    width = 4 * nh
+    assert info["node_bytes"] == 128 * nh and width in (4, 8)
+    halves = oracle_c1.nodes().view(np.uint8).reshape(-1, nh, 128)
+    f = np.concatenate([halves[:, h, :96].copy().view(np.float32).reshape(-1, 6, 4) for h in range(nh)], axis=2)
+    child = np.concatenate([halves[:, h, 96:112].copy().view(np.int32).reshape(-1, 4) for h in range(nh)], axis=1)
+    assert np.all(halves[:, :, 112:] == 0)
     ids, codes = oracle_c1.order()
     tri = config1["scene"].tri_pos().reshape(-1, 3, 3)
     # sort order: permutation, sorted by (code, id), codes recomputed in float64
@@ -399,7 +404,7 @@ def test_lbvh_invariants(config1, oracle_c1):
     stack = [(0, None)]
     while stack:
         w, anc = stack.pop()
-        for s in range(4):
+        for s in range(width):
             c = int(child[w, s])
             if c == 0x7FFFFFFF:
                 continue
@@ -415,7 +420,12 @@ def test_lbvh_invariants(config1, oracle_c1):
                 assert c > w
                 stack.append((c, boxes))
     assert np.all(covered == 1)
-    assert info["stack_bound"] <= 3 * info["levels"]
+    # slots fill in order, empty slots hold the far box
+    filled = child != 0x7FFFFFFF
+    assert np.all(filled[:, :-1] | ~filled[:, 1:])
+    assert np.all(f[:, :, :][~np.repeat(filled[:, None, :], 6, axis=1)] == np.float32(3.0e38))
+    # a node pushes at most width - 1 children per level on the way down
+    assert info["stack_bound"] <= (width - 1) * info["levels"]
 
 
 # ---------------------------------------------------------- convergence ----
