@@ -136,8 +136,10 @@ def rms(a, b):
 
 def test_learned_sampler_gain(lib, config1):
     """Trained on held-out poses, the network's allocation reaches the RMS error
-    of uniform sampling with fewer samples (the method without the network is
-    uniform sampling at the same mean error)."""
+    of uniform sampling with fewer samples, and with fewer than the network-free
+    allocation from the same pilots (pooled pilot standard deviation of the
+    unit and its neighbours), each rated by the uniform samples its error
+    needs."""
     load_c1(lib, config1)
     scene = config1["scene"]
     train_poses = scenes.batch_poses(scene, 96, seed=101, extent=10.0)
@@ -189,3 +191,5 @@ def test_learned_sampler_gain(lib, config1):
     gain_nf = need_nf / per_nf
     print(f"network-free error {e_nf:.5f} with {per_nf:.1f} spp; uniform needs {need_nf:.1f} spp; "
           f"gain {gain_nf:.2f}x; learned over network-free {gain / gain_nf:.2f}x")
+    # measured on MI355X: 2.34x against 1.99x (1.18x); the network must pay for itself
+    assert gain > 1.05 * gain_nf, (gain, gain_nf)
