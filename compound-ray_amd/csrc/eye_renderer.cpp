@@ -51,7 +51,9 @@ struct EyeState {
   size_t d_sample_offsets_cap = 0;
   uint32_t* d_order = nullptr;           // processing order: ommatidia sorted by axis direction
   size_t d_order_cap = 0;
-  bool device_dirty = true;
+  bool omm_dirty = true;      // ommatidia changed: upload them, recompute the processing order
+  bool counts_dirty = true;   // sample counts changed: upload count and sample-record offsets
+  bool device_dirty() const { return omm_dirty || counts_dirty; }
   float4* d_samples = nullptr;
   size_t d_samples_cap = 0;
   float4* d_colours = nullptr;
@@ -251,75 +253,86 @@ void free_scene() {
   s.loaded = false;
 }
 
+// Uploads what changed since the last render: the ommatidia and their
+// processing order (host sort) only when the ommatidia changed, the count
+// and sample-record offsets when the sample counts (or the uniform rate)
+// changed.
 void upload_eye(EyeState& e) {
   State& s = st();
-  if (!e.device_dirty) return;
-  ensure_device(&e.d_omms, &e.d_omms_cap, e.omms.size());
-  if (!e.omms.empty())
-    hip_check(hipMemcpyAsync(e.d_omms, e.omms.data(), sizeof(Ommatidium) * e.omms.size(), hipMemcpyHostToDevice,
-                             s.stream),
-              "upload ommatidia");
+  if (!e.device_dirty()) return;
   const size_t n = e.omms.size();
-  std::vector<uint64_t> first(n);
-  if (!e.counts.empty()) {
-    ensure_device(&e.d_offsets, &e.d_offsets_cap, e.offsets.size());
-    hip_check(hipMemcpyAsync(e.d_offsets, e.offsets.data(), sizeof(uint32_t) * e.offsets.size(),
-                             hipMemcpyHostToDevice, s.stream),
-              "upload offsets");
-    ensure_device(&e.d_counts, &e.d_counts_cap, n);
-    hip_check(hipMemcpyAsync(e.d_counts, e.counts.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice, s.stream),
-              "upload counts");
-    for (size_t i = 0; i < n; ++i) first[i] = e.offsets[i];
-  } else {
-    for (size_t i = 0; i < n; ++i) first[i] = (uint64_t)i * (uint64_t)e.spp;
-  }
-  // Units are processed in the order of (position cell, axis Morton code):
-  // waves running at the same time trace neighbouring view directions from
-  // the same place and share BVH nodes in cache.  Cells are 1 scene unit (or
-  // 1/1024 of the positions' extent, if larger), so one eye (millimetres
-  // across) sorts by direction alone while eyes merged into one ommatidium
-  // list stay grouped.  Output positions are unchanged.
-  float plo[3] = {INFINITY, INFINITY, INFINITY}, phi[3] = {-INFINITY, -INFINITY, -INFINITY};
-  for (size_t i = 0; i < n; ++i) {
-    const float p[3] = {e.omms[i].px, e.omms[i].py, e.omms[i].pz};
+  std::vector<uint32_t> order;
+  if (e.omm_dirty) {
+    ensure_device(&e.d_omms, &e.d_omms_cap, n);
+    if (n)
+      hip_check(hipMemcpyAsync(e.d_omms, e.omms.data(), sizeof(Ommatidium) * n, hipMemcpyHostToDevice, s.stream),
+                "upload ommatidia");
+    // Units are processed in the order of (position cell, axis Morton code):
+    // waves running at the same time trace neighbouring view directions from
+    // the same place and share BVH nodes in cache.  Cells are 1 scene unit (or
+    // 1/1024 of the positions' extent, if larger), so one eye (millimetres
+    // across) sorts by direction alone while eyes merged into one ommatidium
+    // list stay grouped.  Output positions are unchanged.
+    float plo[3] = {INFINITY, INFINITY, INFINITY}, phi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (size_t i = 0; i < n; ++i) {
+      const float p[3] = {e.omms[i].px, e.omms[i].py, e.omms[i].pz};
+      for (int a = 0; a < 3; ++a)
+        if (std::isfinite(p[a])) {
+          plo[a] = std::min(plo[a], p[a]);
+          phi[a] = std::max(phi[a], p[a]);
+        }
+    }
+    double ext = 0.0;
     for (int a = 0; a < 3; ++a)
-      if (std::isfinite(p[a])) {
-        plo[a] = std::min(plo[a], p[a]);
-        phi[a] = std::max(phi[a], p[a]);
-      }
+      if (phi[a] >= plo[a]) ext = std::max(ext, (double)phi[a] - (double)plo[a]);
+    const double cell = std::max(1.0, ext / 1023.0);
+    auto cell_of = [&](float p, int a) -> uint32_t {
+      if (!std::isfinite(p) || !(phi[a] >= plo[a])) return 0u;
+      const double q = std::floor(((double)p - (double)plo[a]) / cell);
+      return (uint32_t)std::min(1023.0, std::max(0.0, q));
+    };
+    std::vector<std::pair<uint64_t, uint32_t>> key(n);
+    for (size_t i = 0; i < n; ++i) {
+      const V3 d = normalize(v3(e.omms[i].dx, e.omms[i].dy, e.omms[i].dz));
+      const float lo[3] = {-1.0f, -1.0f, -1.0f}, hi[3] = {1.0f, 1.0f, 1.0f};
+      const bool ok = d.x == d.x && d.y == d.y && d.z == d.z;
+      const uint64_t where = ((uint64_t)expand_bits10(cell_of(e.omms[i].px, 0)) << 2) |
+                             ((uint64_t)expand_bits10(cell_of(e.omms[i].py, 1)) << 1) |
+                             (uint64_t)expand_bits10(cell_of(e.omms[i].pz, 2));
+      key[i] = {(where << 30) | (ok ? morton30(d.x, d.y, d.z, lo, hi) : 0u), (uint32_t)i};
+    }
+    std::sort(key.begin(), key.end());
+    order.resize(n);
+    for (size_t i = 0; i < n; ++i) order[i] = key[i].second;
+    ensure_device(&e.d_order, &e.d_order_cap, n);
+    if (n)
+      hip_check(hipMemcpyAsync(e.d_order, order.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice, s.stream),
+                "upload order");
   }
-  double ext = 0.0;
-  for (int a = 0; a < 3; ++a)
-    if (phi[a] >= plo[a]) ext = std::max(ext, (double)phi[a] - (double)plo[a]);
-  const double cell = std::max(1.0, ext / 1023.0);
-  auto cell_of = [&](float p, int a) -> uint32_t {
-    if (!std::isfinite(p) || !(phi[a] >= plo[a])) return 0u;
-    const double q = std::floor(((double)p - (double)plo[a]) / cell);
-    return (uint32_t)std::min(1023.0, std::max(0.0, q));
-  };
-  std::vector<std::pair<uint64_t, uint32_t>> key(n);
-  for (size_t i = 0; i < n; ++i) {
-    const V3 d = normalize(v3(e.omms[i].dx, e.omms[i].dy, e.omms[i].dz));
-    const float lo[3] = {-1.0f, -1.0f, -1.0f}, hi[3] = {1.0f, 1.0f, 1.0f};
-    const bool ok = d.x == d.x && d.y == d.y && d.z == d.z;
-    const uint64_t where = ((uint64_t)expand_bits10(cell_of(e.omms[i].px, 0)) << 2) |
-                           ((uint64_t)expand_bits10(cell_of(e.omms[i].py, 1)) << 1) |
-                           (uint64_t)expand_bits10(cell_of(e.omms[i].pz, 2));
-    key[i] = {(where << 30) | (ok ? morton30(d.x, d.y, d.z, lo, hi) : 0u), (uint32_t)i};
-  }
-  std::sort(key.begin(), key.end());
-  std::vector<uint32_t> order(n);
-  for (size_t i = 0; i < n; ++i) order[i] = key[i].second;
-  ensure_device(&e.d_order, &e.d_order_cap, n);
-  ensure_device(&e.d_sample_offsets, &e.d_sample_offsets_cap, n);
-  if (n) {
-    hip_check(hipMemcpyAsync(e.d_order, order.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice, s.stream),
-              "upload order");
-    hip_check(hipMemcpyAsync(e.d_sample_offsets, first.data(), sizeof(uint64_t) * n, hipMemcpyHostToDevice, s.stream),
-              "upload sample offsets");
+  std::vector<uint64_t> first;
+  if (e.counts_dirty || e.omm_dirty) {
+    first.resize(n);
+    if (!e.counts.empty()) {
+      ensure_device(&e.d_offsets, &e.d_offsets_cap, e.offsets.size());
+      hip_check(hipMemcpyAsync(e.d_offsets, e.offsets.data(), sizeof(uint32_t) * e.offsets.size(),
+                               hipMemcpyHostToDevice, s.stream),
+                "upload offsets");
+      ensure_device(&e.d_counts, &e.d_counts_cap, n);
+      hip_check(hipMemcpyAsync(e.d_counts, e.counts.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice, s.stream),
+                "upload counts");
+      for (size_t i = 0; i < n; ++i) first[i] = e.offsets[i];
+    } else {
+      for (size_t i = 0; i < n; ++i) first[i] = (uint64_t)i * (uint64_t)e.spp;
+    }
+    ensure_device(&e.d_sample_offsets, &e.d_sample_offsets_cap, n);
+    if (n)
+      hip_check(hipMemcpyAsync(e.d_sample_offsets, first.data(), sizeof(uint64_t) * n, hipMemcpyHostToDevice,
+                               s.stream),
+                "upload sample offsets");
   }
   hip_check(hipStreamSynchronize(s.stream), "sync");
-  e.device_dirty = false;
+  e.omm_dirty = false;
+  e.counts_dirty = false;
 }
 
 EyeLaunch launch_of(EyeState& e, const Pose* d_poses, int pose_count) {
@@ -831,9 +844,11 @@ bool isCompoundEyeActive(void) { return current_eye() != nullptr; }
 void setCurrentEyeSamplesPerOmmatidium(int s) {
   EyeState* e = current_eye();
   if (!e) return;
-  e->spp = std::max(1, s);
+  const int spp = std::max(1, s);
+  // per-sample record offsets depend on the rate (with uniform counts)
+  if (spp != e->spp && e->counts.empty()) e->counts_dirty = true;
+  e->spp = spp;
   e->restart_streams();
-  e->device_dirty = true;   // per-sample record offsets depend on the count
 }
 
 int getCurrentEyeSamplesPerOmmatidium(void) {
@@ -866,7 +881,7 @@ void setOmmatidia(OmmatidiumPacket* omms, size_t count) {
   e->omms.resize(count);
   e->omm_version++;
   if (count) memcpy(e->omms.data(), omms, sizeof(Ommatidium) * count);
-  e->device_dirty = true;
+  e->omm_dirty = true;
   ABI_GUARD_END(return)
 }
 
@@ -975,7 +990,7 @@ void setCurrentEyeSampleCounts(const uint32_t* counts, size_t count) {
     recompute_offsets(*e);
   }
   e->restart_streams();
-  e->device_dirty = true;
+  e->counts_dirty = true;
   ABI_GUARD_END(return)
 }
 
@@ -1227,7 +1242,7 @@ int renderPoseBatchInto(const float* devPoses, size_t poseCount, float* devColou
   if (poseCount > (size_t)INT32_MAX || units > kNoAllocMaxUnits)
     throw std::runtime_error("renderPoseBatchInto: " + std::to_string(units) + " units exceed the maximum of " +
                              std::to_string(kNoAllocMaxUnits) + " per call");
-  if (e->device_dirty)
+  if (e->device_dirty())
     throw std::runtime_error(
         "renderPoseBatchInto: the current eye's ommatidia or sample counts changed since they were last uploaded; "
         "render the eye once with renderFrame or renderPoseBatch first (those calls may allocate)");
