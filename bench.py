@@ -176,6 +176,24 @@ def calibrate_budget(err_at, target: float, iters: int = 6):
     return best if best is not None else last
 
 
+def neyman_mean_spp(nreq, floor: float = 1.0, cap: float = 4096.0) -> float:
+    """Smallest mean sample count of any allocation n_i in [floor, cap] that
+    meets the RMS target, given each unit's required samples nreq_i (its
+    variance over the squared target): the mean squared error is
+    mean(nreq_i / n_i) <= 1, minimised in total by n_i proportional to
+    sqrt(nreq_i) (Neyman) and clipped to the bounds; the multiplier is found
+    by bisection."""
+    n = np.asarray(nreq, np.float64)
+    lo, hi = 1e-9, 1e9
+    for _ in range(200):
+        lam = math.sqrt(lo * hi)
+        if np.mean(n / np.clip(lam * np.sqrt(n), floor, cap)) <= 1.0:
+            hi = lam
+        else:
+            lo = lam
+    return float(np.clip(hi * np.sqrt(n), floor, cap).mean())
+
+
 def bench_adaptive(lib, er, scenes, cfg, eyes, target: float = 0.01, n_test: int = 1024, n_train: int = 8192,
                    train_units: int = 2_000_000, label_spp: int = 1024, n_ref: int = 16) -> dict:
     """Learned adaptive sampling on the bee-scale scene and eyes: the sampler
@@ -548,6 +566,17 @@ def bench_large(lib, er, scenes, tmp: str, steps: int, warmup: int, target: floa
     bytes_launch = b_packet / 2
     achieved = bytes_launch / (launch_ms * 1e-3) / 1e9
     traffic = measured_traffic("k_render_units", "c5")
+    # the most an allocation could save with exact knowledge of every unit's
+    # variance: required samples of a seeded tenth of the test batch's units
+    # from 1,024 samples each, against the uniform rate they imply
+    lib.setCurrentEyeFrameIndex(99)
+    bmask = (np.random.default_rng(905).random((1, n_omm)) < 0.1).astype(np.uint8)
+    _, blab = er.training_set_np(lib, identity, unit_mask=bmask, pilot=8, label_spp=1024, error_target=target)
+    nreq = np.exp2(blab.astype(np.float64))
+    bound = {"units": int(len(nreq)), "uniform_spp": float(nreq.mean()),
+             "ideal_mean_spp": neyman_mean_spp(nreq), "ideal_mean_spp_pilot_floor": neyman_mean_spp(nreq, floor=8.0)}
+    bound["ideal_sample_ratio"] = bound["uniform_spp"] / bound["ideal_mean_spp"]
+    bound["ideal_sample_ratio_pilot_floor"] = bound["uniform_spp"] / bound["ideal_mean_spp_pilot_floor"]
     out = {
         "workload": f"{len(eyes)} eyes ({n_omm} ommatidia, log-uniform 1k-30k per eye), each at its own pose, "
                     f"{scene.tri_count()} triangles (2048^2 heightfield + 20,000 cylinders + 200 rocks), "
@@ -585,6 +614,7 @@ def bench_large(lib, er, scenes, tmp: str, steps: int, warmup: int, target: floa
                      "per_ray_equivalent_GBps": b_per_ray / 2 / (launch_ms * 1e-3) / 1e9,
                      "hbm_frac_measured": None if traffic is None else traffic / (launch_ms * 1e-3) / 1e9 /
                      HBM_PEAK_GBS},
+        "sampling_bound": bound,
         "launch_shape": shape,
         "bvh": bvh,
         "scene_load_s": load_s,

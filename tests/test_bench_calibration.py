@@ -57,3 +57,20 @@ def test_smallest_scale_meeting_the_target(bench):
     scale, e, _ = bench.calibrate_budget(err_at, 0.01)
     assert e <= 0.01
     assert scale < 1.0
+
+
+def test_neyman_bound_closed_forms(bench):
+    """Equal needs: the allocation is uniform.  Two halves needing a and b
+    samples: Neyman gives mean (sqrt a + sqrt b)^2 / 4 against (a + b) / 2
+    uniform.  A floor above the optimum of the small half raises the mean to
+    what the large half then needs on top of the floor."""
+    import numpy as np
+
+    assert abs(bench.neyman_mean_spp(np.full(100, 37.0)) - 37.0) < 1e-6
+    a, b = 4.0, 100.0
+    n = np.array([a] * 50 + [b] * 50)
+    assert abs(bench.neyman_mean_spp(n) - (a ** 0.5 + b ** 0.5) ** 2 / 4) < 1e-6
+    # without a floor the small half gets 12 samples; with a floor of 20 it gets
+    # 20 (error share 4 / 20 = 0.2), and the large half needs 100 / n_b = 1.8
+    m = bench.neyman_mean_spp(n, floor=20.0)
+    assert abs(m - (20.0 + 100.0 / 1.8) / 2) < 1e-6

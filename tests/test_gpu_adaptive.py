@@ -34,6 +34,11 @@ def test_adaptive_pipeline_exact(lib, config1, oracle_c1):
     assert ms >= 0
     assert counts.min() >= 8 and counts.max() <= 96
     assert total == int(counts.sum())
+    # top-ups fill their lane footprint: 0, a power of two up to 32, or whole
+    # 64-lane rounds (clamped by the maximum)
+    e = counts.astype(np.int64) - 8
+    fills = np.isin(e, [0, 1, 2, 4, 8, 16, 32]) | ((e % 64) == 0) | (counts == 96)
+    assert np.all(fills), np.unique(e)
     poses12 = np.stack([oracle.pose12(p) for p in poses6])
     eye = lib.getCurrentEyeIndex()
     frame = lib.getCurrentEyeFrameIndex()
