@@ -157,6 +157,46 @@ def test_split_launch_routing(lib, config1, oracle_c1, mode, spp):
         lib.setSampleRecording(True)
 
 
+def test_eye_changes_between_renders(lib, config1, oracle_c1):
+    """One eye re-rendered after each kind of change: the sample rate (up and
+    down), uneven counts, back to a uniform rate, new ommatidia of another
+    count, and the same ommatidia again.  The library uploads only what
+    changed (the ommatidia and their processing order, or the count and
+    sample-record offsets); every render's samples and colours equal the
+    oracle's."""
+    rng = np.random.default_rng(31)
+    omm_a = config1["eyes"][0][:240]
+    omm_b = config1["eyes"][0][300:557]
+    setup_eye(lib, config1, omm=omm_a, spp=32)
+
+    def check(omm, spp=None, counts=None):
+        assert lib.renderFrame() >= 0, er.last_error(lib)
+        frame = lib.getCurrentEyeFrameIndex() - 1
+        ref = oracle_c1.render(omm, er.pose_np(lib), spp=spp or 1, counts=counts, eye=lib.getCurrentEyeIndex(),
+                               frame=frame, want_samples=True)
+        assert np.array_equal(er.samples_np(lib).view(np.uint32), ref["samples"].view(np.uint32))
+        assert np.array_equal(er.colours_np(lib), ref["colour_gpu_order"][0])
+
+    check(omm_a, spp=32)
+    lib.setCurrentEyeSamplesPerOmmatidium(7)
+    check(omm_a, spp=7)
+    lib.setCurrentEyeSamplesPerOmmatidium(100)
+    check(omm_a, spp=100)
+    lib.setCurrentEyeSamplesPerOmmatidium(100)   # unchanged rate
+    check(omm_a, spp=100)
+    counts = rng.integers(1, 90, size=len(omm_a)).astype(np.uint32)
+    er.set_sample_counts(lib, counts)
+    check(omm_a, counts=counts)
+    er.set_sample_counts(lib, None)
+    lib.setCurrentEyeSamplesPerOmmatidium(16)
+    check(omm_a, spp=16)
+    er.set_ommatidia_np(lib, omm_b)
+    check(omm_b, spp=16)
+    er.set_ommatidia_np(lib, omm_a)
+    lib.setCurrentEyeSamplesPerOmmatidium(65)
+    check(omm_a, spp=65)
+
+
 def test_unrecorded_render_matches(lib, config1, oracle_c1):
     """Default renderFrame path (samples reduced in registers, nothing recorded)."""
     setup_eye(lib, config1, spp=48, record=False)
