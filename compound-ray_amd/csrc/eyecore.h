@@ -203,10 +203,12 @@ EYE_HD RayPre precompute_ray(const Ray& r) {
   return p;
 }
 
-// Returns true and (t, b1, b2) when the ray hits the triangle with
-// tmin <= t <= tmax.  b1, b2 weight vertices 1 and 2 (OptiX convention).
-EYE_HD bool intersect_triangle(const RayPre& r, V3 v0, V3 v1, V3 v2, float* t_out,
-                               float* b1_out, float* b2_out) {
+// Returns true and (t, V, W, det) when the ray hits the triangle with
+// tmin <= t <= tmax; the barycentrics are b1 = V / det and b2 = W / det.
+// Traversal keeps (V, W, det) of the closest hit and divides once at the end:
+// the same operations on the same values as dividing at every hit.
+EYE_HD bool intersect_triangle_raw(const RayPre& r, V3 v0, V3 v1, V3 v2, float* t_out, float* v_out,
+                                   float* w_out, float* det_out) {
   const V3 A = v3(v0.x - r.ox, v0.y - r.oy, v0.z - r.oz);
   const V3 B = v3(v1.x - r.ox, v1.y - r.oy, v1.z - r.oz);
   const V3 C = v3(v2.x - r.ox, v2.y - r.oy, v2.z - r.oz);
@@ -229,6 +231,18 @@ EYE_HD bool intersect_triangle(const RayPre& r, V3 v0, V3 v1, V3 v2, float* t_ou
   const float t = T / det;
   if (!(t >= r.tmin && t <= r.tmax)) return false;
   *t_out = t;
+  *v_out = V;
+  *w_out = W;
+  *det_out = det;
+  return true;
+}
+
+// Returns true and (t, b1, b2) when the ray hits the triangle with
+// tmin <= t <= tmax.  b1, b2 weight vertices 1 and 2 (OptiX convention).
+EYE_HD bool intersect_triangle(const RayPre& r, V3 v0, V3 v1, V3 v2, float* t_out,
+                               float* b1_out, float* b2_out) {
+  float V, W, det;
+  if (!intersect_triangle_raw(r, v0, v1, v2, t_out, &V, &W, &det)) return false;
   *b1_out = V / det;
   *b2_out = W / det;
   return true;

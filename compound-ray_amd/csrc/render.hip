@@ -46,7 +46,16 @@ struct Hit {
   float t, b1, b2;
   uint32_t k;   // sorted triangle position
   uint32_t id;  // original triangle id, 0xffffffff for a miss
+  float det;    // during traversal b1, b2 hold V, W of the closest hit and det its
+                // determinant; finish_hit divides them (b1 = V / det, b2 = W / det)
 };
+
+__device__ __forceinline__ void finish_hit(Hit& h) {
+  if (h.id != 0xffffffffu) {
+    h.b1 = h.b1 / h.det;
+    h.b2 = h.b2 / h.det;
+  }
+}
 
 // Pointers into the constant address space: wave-uniform loads through them
 // become scalar (SMEM) loads, fetched once per wave into SGPRs.  Only used for
@@ -81,13 +90,14 @@ __device__ __forceinline__ void test_leaf(P4 tris, int32_t code, const RayPre& r
     const float4 a = load4(tris, 3 * (size_t)k + 0);
     const float4 b = load4(tris, 3 * (size_t)k + 1);
     const float4 c = load4(tris, 3 * (size_t)k + 2);
-    float t, b1, b2;
-    if (intersect_triangle(rp, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), v3(c.x, c.y, c.z), &t, &b1, &b2)) {
+    float t, vv, ww, det;
+    if (intersect_triangle_raw(rp, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), v3(c.x, c.y, c.z), &t, &vv, &ww, &det)) {
       const uint32_t id = __float_as_uint(a.w);
       if (t < h.t || (t == h.t && id < h.id)) {
         h.t = t;
-        h.b1 = b1;
-        h.b2 = b2;
+        h.b1 = vv;
+        h.b2 = ww;
+        h.det = det;
         h.k = k;
         h.id = id;
       }
@@ -124,12 +134,13 @@ __device__ __forceinline__ void test_leaf_wave(const float4* tris, int32_t code,
       const V3 c = v3(__int_as_float(__builtin_amdgcn_readlane(v, o + 8)),
                       __int_as_float(__builtin_amdgcn_readlane(v, o + 9)),
                       __int_as_float(__builtin_amdgcn_readlane(v, o + 10)));
-      float t, b1, b2;
-      if (intersect_triangle(rp, a, b, c, &t, &b1, &b2)) {
+      float t, vv, ww, det;
+      if (intersect_triangle_raw(rp, a, b, c, &t, &vv, &ww, &det)) {
         if (t < h.t || (t == h.t && id < h.id)) {
           h.t = t;
-          h.b1 = b1;
-          h.b2 = b2;
+          h.b1 = vv;
+          h.b2 = ww;
+          h.det = det;
           h.k = first + q0 + q;
           h.id = id;
         }
@@ -236,6 +247,7 @@ __device__ __forceinline__ void closest_hit(const SceneView& S, const RayPre& rp
     closest_hit_impl<kStats, true>(S, rp, stack, stride, h, nodes_visited, tris_tested);
   else
     closest_hit_impl<kStats, false>(S, rp, stack, stride, h, nodes_visited, tris_tested);
+  finish_hit(h);
 }
 
 __device__ __forceinline__ V3 shade(const SceneView& S, const Hit& h, float dx, float dy, float dz) {
@@ -416,6 +428,7 @@ __device__ __forceinline__ void closest_hit_packet(const SceneView& S, const Ray
     closest_hit_packet_impl<true, kScalar>(S, rp, active, h, steps, tris);
   else
     closest_hit_packet_impl<false, kScalar>(S, rp, active, h, steps, tris);
+  finish_hit(h);
 }
 
 // The value of lane (lane ^ M), M a power of two below 64, without an
