@@ -65,6 +65,18 @@ struct EyeState {
   size_t d_chunk_partial_cap = 0;
   uint32_t* d_chunk_done = nullptr;
   size_t d_chunk_done_cap = 0;
+  // long-task split of split renders: per task the packet steps of the last
+  // render and a flag, the split list and counters (render.h, UnitLaunch)
+  uint32_t* d_split = nullptr;
+  size_t d_split_cap = 0;
+  uint64_t split_tasks = 0;       // tasks the buffers are laid out for
+  bool split_valid = false;       // the cost words hold the steps of a render of those tasks
+  uint32_t split_last = 0;        // tasks split by the last render
+  uint64_t split_generation = 0;  // scene the costs were measured on
+  // automatic choice: renders of the same tasks since the last trial began,
+  // and the shortest trace time (ms) of the trial's renders without / with
+  uint32_t split_frames = 0;
+  float split_ms[2] = {1e30f, 1e30f};
   // pixel -> ommatidium map of the spherical projections, valid for
   // (proj_w, proj_h, proj_variant) and the ommatidia of proj_version
   uint32_t* d_proj_ids = nullptr;
@@ -242,6 +254,7 @@ void free_scene() {
     if (e.d_proj_ids) (void)hipFree(e.d_proj_ids);
     if (e.d_chunk_partial) (void)hipFree(e.d_chunk_partial);
     if (e.d_chunk_done) (void)hipFree(e.d_chunk_done);
+    if (e.d_split) (void)hipFree(e.d_split);
     if (e.d_counts) (void)hipFree(e.d_counts);
     if (e.d_sample_offsets) (void)hipFree(e.d_sample_offsets);
     if (e.d_order) (void)hipFree(e.d_order);
@@ -251,6 +264,51 @@ void free_scene() {
   s.cams.clear();
   s.current = 0;
   s.loaded = false;
+}
+
+// Long-task split of the packet task kernel (render.h, UnitLaunch::split_cost):
+// 0 never, 1 always, 2 (default) chosen per eye by measurement.  Each trial
+// traces renders 1 and 2 of the same tasks without it, render 3 records the
+// tasks' steps, renders 4 and 5 trace with it; the faster way (by the
+// shorter of each pair's trace times, with it only if 3% faster) is kept
+// until render kSplitTrialEvery, when a new trial begins.  Any change of the
+// rays (ommatidia, rate, scene) starts a trial over.  Results do not depend
+// on it.
+int g_task_split = 2;
+constexpr uint32_t kSplitTrialEvery = 1000;
+
+void split_restart(EyeState& e) {
+  e.split_valid = false;
+  e.split_frames = 0;
+  e.split_ms[0] = e.split_ms[1] = 1e30f;
+}
+
+// Whether this render of e records its tasks' steps, and whether it also
+// plans the split from the last render's steps.
+void split_choice(const EyeState& e, bool* record, bool* plan) {
+  bool on = false, rec = false;
+  if (g_task_split == 1) {
+    on = rec = true;
+  } else if (g_task_split == 2) {
+    const uint32_t k = e.split_frames;
+    rec = k >= 3 && (k <= 5 || e.split_ms[1] < 0.97f * e.split_ms[0]);
+    on = rec && k != 3;
+  }
+  *record = rec;
+  *plan = on && e.split_valid;
+}
+
+// After a render through the task kernel: its trace time goes to the trial.
+void split_account(EyeState& e, float trace_ms) {
+  const uint32_t k = e.split_frames;
+  if (k == 1 || k == 2) e.split_ms[0] = std::min(e.split_ms[0], trace_ms);
+  if (k == 4 || k == 5) e.split_ms[1] = std::min(e.split_ms[1], trace_ms);
+  if (k + 1 >= kSplitTrialEvery) {
+    e.split_frames = 1;   // next trial: the buffers' costs stay valid
+    e.split_ms[0] = e.split_ms[1] = 1e30f;
+  } else {
+    e.split_frames = k + 1;
+  }
 }
 
 // Uploads what changed since the last render: the ommatidia and their
@@ -333,6 +391,7 @@ void upload_eye(EyeState& e) {
   hip_check(hipStreamSynchronize(s.stream), "sync");
   e.omm_dirty = false;
   e.counts_dirty = false;
+  split_restart(e);   // other rays: the last render's task costs no longer apply
 }
 
 EyeLaunch launch_of(EyeState& e, const Pose* d_poses, int pose_count) {
@@ -427,8 +486,46 @@ double render_compound(EyeState& e, const Cam& c) {
     U.chunk_partial_cap = e.d_chunk_partial_cap;
     U.chunk_done = e.d_chunk_done;
     U.chunk_done_cap = e.d_chunk_done_cap;
+    // the packet task kernel's long-task split, planned from the last render
+    const uint64_t tasks = (uint64_t)e.omms.size() * chunks;
+    if (e.split_tasks != tasks || e.split_generation != s.scene.generation) {
+      split_restart(e);
+      e.split_tasks = tasks;
+      e.split_generation = s.scene.generation;
+    }
+    bool rec = false, plan = false;
+    split_choice(e, &rec, &plan);
+    if (rec) {
+      const uint32_t* before = e.d_split;
+      ensure_device(&e.d_split, &e.d_split_cap, (size_t)(2 * tasks + 2 * kSplitMax + 2));
+      if (e.d_split != before) {   // new buffer: no steps recorded in it yet
+        e.split_valid = false;
+        plan = false;
+      }
+      U.split_cost = e.d_split;
+      U.split_flag = U.split_cost + tasks;
+      U.split_list = U.split_flag + tasks;
+      U.split_count = U.split_list + kSplitMax;
+      U.split_sub_done = U.split_count + 2;
+      U.split_tasks = tasks;
+      U.split_plan = plan;
+    }
   }
-  if (!launch_render_units(s.scene, U, s.stream, &err)) throw std::runtime_error(err);
+  if (!launch_render_units(s.scene, U, s.stream, &err)) {
+    split_restart(e);
+    throw std::runtime_error(err);
+  }
+  e.split_last = 0;
+  const bool tasks_kernel = std::strcmp(last_render_kernel(), "k_render_tasks") == 0;
+  if (U.split_cost && tasks_kernel) {
+    if (U.split_plan)
+      hip_check(hipMemcpyAsync(&e.split_last, U.split_count, sizeof(uint32_t), hipMemcpyDeviceToHost, s.stream),
+                "split count");
+    e.split_valid = true;
+  } else {
+    e.split_valid = false;
+  }
+  if (!tasks_kernel) split_restart(e);
   hip_check(hipEventRecord(s.ev[1], s.stream), "event");
   hip_check(hipEventRecord(s.ev[2], s.stream), "event");
   bool ok = true;
@@ -468,6 +565,7 @@ double render_compound(EyeState& e, const Cam& c) {
   s.timings[1] = f[1];
   s.timings[2] = f[2];
   s.timings[3] = (double)f[0] + f[1] + f[2];
+  if (e.counts.empty() && std::strcmp(last_render_kernel(), "k_render_tasks") == 0) split_account(e, f[0]);
   e.frame += 1;
   e.has_render = true;
   e.rendered_rays = record ? total : 0;
@@ -1559,6 +1657,22 @@ void setSampleRecording(bool enabled) { st().record_samples = enabled; }
 const char* getLastRenderKernel(void) { return last_render_kernel(); }
 
 void setSplitLaunches(bool enabled) { set_split_launches(enabled); }
+
+void setTaskSplit(int mode) {
+  if (mode < 0 || mode > 2) {
+    set_error("setTaskSplit: mode must be 0 (never), 1 (always) or 2 (by measurement)");
+    return;
+  }
+  if (mode != g_task_split) {
+    g_task_split = mode;
+    for (auto& e : st().eyes) split_restart(e);
+  }
+}
+
+int getCurrentEyeTaskSplits(void) {
+  EyeState* e = current_eye();
+  return e ? (int)std::min<uint32_t>(e->split_last, (uint32_t)kSplitMax) : 0;
+}
 
 void setRenderTuning(int unitsPerGrab, int blocks) {
   State& s = st();

@@ -26,6 +26,13 @@ struct EyeLaunch {
   uint32_t frame = 0;
 };
 
+// Long-task split of the packet task kernel (see UnitLaunch::split_cost).
+constexpr int kSplitWays = 2;             // packets a split task is traced as             // packets a split task is traced as
+constexpr int kSplitMax = 1024;           // tasks split per launch at most            // tasks split per launch at most
+constexpr uint32_t kSplitMinSteps = 128;  // a task that took fewer packet steps is never split  // a task that took fewer packet steps is never split
+constexpr int kSplitBins = 256;           // threshold search: histogram bins of 8 steps
+static_assert(64 % kSplitWays == 0, "a split packet traces 64 / kSplitWays lanes");
+
 // Persistent ommatidium-unit renderer.  A unit is one ommatidium at one pose;
 // waves take units from a work queue (`units_per_grab` at a time), trace a
 // unit's samples 64 at a time as one wave packet and reduce them in registers
@@ -56,6 +63,22 @@ struct UnitLaunch {
   uint64_t chunk_partial_cap = 0;       // float4 elements
   uint32_t* chunk_done = nullptr;
   uint64_t chunk_done_cap = 0;          // counters
+  // Optional long-task split of the packet task kernel (an eye rendered frame
+  // after frame, one pose, uniform counts).  The packet steps each task took
+  // in the last launch (split_cost, per task) predict this launch's longest
+  // tasks: up to kSplitMax of them, at least kSplitMinSteps long, are traced
+  // as kSplitWays packets of 64 / kSplitWays rays of neighbouring elevation,
+  // started before every other task.  Results do not depend on it.  Device
+  // buffers: split_cost and split_flag (split_tasks entries each), split_list
+  // and split_sub_done (kSplitMax), split_count (2).  split_plan: split_cost
+  // holds the steps of a launch of the same tasks.
+  uint32_t* split_cost = nullptr;
+  uint32_t* split_flag = nullptr;
+  uint32_t* split_list = nullptr;
+  uint32_t* split_count = nullptr;      // [0] tasks split, [1] threshold in steps
+  uint32_t* split_sub_done = nullptr;
+  uint64_t split_tasks = 0;
+  bool split_plan = false;
   // launch shape (results do not depend on it)
   int units_per_grab = 0;               // consecutive units a wave takes from the queue (0: chosen
                                         //   from the launch size, at most auto_grab_max)

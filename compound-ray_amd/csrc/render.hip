@@ -860,14 +860,32 @@ __global__ __launch_bounds__(kTraceThreads, 5) void k_render_chunks(SceneView S,
 // between the ray generation and the chunk store, which keeps register use
 // low (more resident waves).  Same per-lane chunk values, completion
 // counters and last-wave reduction as k_render_chunks (no statistics).
-template <bool kSamples, bool kScalar>
+template <bool kSamples, bool kScalar, bool kSplit>
 __global__ __launch_bounds__(kTraceThreads, 6) void k_render_tasks(SceneView S, UnitLaunch U, int chunks) {
   const int lane = threadIdx.x & 63;
   const uint64_t units = (uint64_t)U.pose_count * (uint64_t)U.omm_count;
   const uint64_t tasks = units * (uint64_t)chunks;
-  const uint64_t t = (uint64_t)xcd_logical_block(blockIdx.x, gridDim.x) * (kTraceThreads / 64) +
-                     (uint64_t)(threadIdx.x >> 6);
-  if (t >= tasks) return;  // whole wave
+  const uint64_t wv = (uint64_t)xcd_logical_block(blockIdx.x, gridDim.x) * (kTraceThreads / 64) +
+                      (uint64_t)(threadIdx.x >> 6);
+  // With a long-task split the first kSplitWays * kSplitMax waves trace the
+  // listed tasks' split packets (wave w: part w % kSplitWays of listed task
+  // w / kSplitWays), the rest every other task in order.
+  const bool split = kSplit && U.split_flag != nullptr;
+  const uint64_t sub_waves = split ? (uint64_t)kSplitWays * (uint64_t)kSplitMax : 0u;
+  uint64_t t;
+  int part = -1;
+  uint32_t li = 0;
+  if (wv < sub_waves) {
+    li = (uint32_t)(wv / (uint64_t)kSplitWays);
+    part = (int)(wv % (uint64_t)kSplitWays);
+    const uint32_t listed = (uint32_t)__builtin_amdgcn_readfirstlane((int)U.split_count[0]);
+    if (li >= listed || li >= (uint32_t)kSplitMax) return;  // whole wave
+    t = (uint32_t)__builtin_amdgcn_readfirstlane((int)U.split_list[li]);
+  } else {
+    t = wv - sub_waves;
+    if (t >= tasks) return;  // whole wave
+    if (split && __builtin_amdgcn_readfirstlane((int)U.split_flag[t]) != 0) return;  // traced split
+  }
   const uint32_t n = (uint32_t)U.spp;
   const float inv = 1.0f / (float)n;
   const uint64_t uq = t / (uint64_t)chunks;
@@ -890,22 +908,45 @@ __global__ __launch_bounds__(kTraceThreads, 6) void k_render_tasks(SceneView S, 
     ray.tmax = 1.0f;
   }
   const RayPre rp = precompute_ray(ray);
+  // A split packet traces the lanes whose rays rank part * 64 / kSplitWays ..
+  // (part + 1) * 64 / kSplitWays - 1 by elevation (the direction's y, ties by
+  // lane): rays of neighbouring elevation fan out less than the whole chunk.
+  bool mine = active;
+  if (part >= 0) {
+    const uint32_t bits = __float_as_uint(ray.dy);
+    const uint32_t key = ((bits & 0x80000000u) ? ~bits : (bits | 0x80000000u)) & ~63u;
+    const uint32_t sorted = wave_sort_desc(key | (uint32_t)lane);
+    // sorted position q holds lane (sorted & 63): send q back to that lane
+    const int rank = __builtin_amdgcn_ds_permute((int)((sorted & 63u) * 4u), lane);
+    mine = active && rank / (64 / kSplitWays) == part;
+  }
   Hit h;
   uint32_t w_steps = 0, w_tris = 0;
-  closest_hit_packet<kScalar>(S, rp, active, h, w_steps, w_tris);
+  closest_hit_packet<kScalar>(S, rp, mine, h, w_steps, w_tris);
   if (U.packet_counts && lane == 0) {
     atomicAdd(U.packet_counts, (unsigned long long)w_steps);
     atomicAdd(U.packet_counts + 1, (unsigned long long)w_tris);
   }
+  if (kSplit && U.split_cost && lane == 0) atomicAdd(U.split_cost + t, w_steps);
   float4 a = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-  if (active) {
+  if (mine) {
     const V3 c = shade(S, h, ray.dx, ray.dy, ray.dz);
     a = make_float4(c.x * inv, c.y * inv, c.z * inv, 0.0f);
     if (kSamples)
       U.samples[U.sample_offsets[out] + s] = make_float4(c.x, c.y, c.z, h.id != 0xffffffffu ? h.t : -1.0f);
   }
-  U.chunk_partial[(uq * (uint64_t)chunks + j) * 64u + (uint64_t)lane] = a;
+  // a split packet writes its own lanes' terms; the other packets of its
+  // task write the rest
+  if (part < 0 || mine) U.chunk_partial[(uq * (uint64_t)chunks + j) * 64u + (uint64_t)lane] = a;
   __threadfence();
+  if (part >= 0) {
+    // the last of the task's split packets to finish completes the task
+    uint32_t done = 0;
+    if (lane == 0) done = atomicAdd(U.split_sub_done + li, 1u);
+    done = __builtin_amdgcn_readfirstlane(__shfl(done, 0));
+    if (done != (uint32_t)kSplitWays - 1u) return;  // whole wave
+    __threadfence();
+  }
   uint32_t prev = 0;
   if (lane == 0) prev = atomicAdd(U.chunk_done + uq, 1u);
   prev = __builtin_amdgcn_readfirstlane(__shfl(prev, 0));
@@ -940,6 +981,68 @@ __global__ __launch_bounds__(64) void k_packet_guard_probe(SceneView S, const Ra
     closest_hit_packet<false>(S, rp, true, h, steps, tris);
   }
   ids[lane] = (int32_t)h.id;
+}
+
+// Long-task split plan, one workgroup launched before the task kernel: it
+// clears the units' completion counters, takes a histogram of the last
+// launch's per-task packet steps (bins of 8 steps, the last one open), picks
+// the smallest step count T >= kSplitMinSteps with at most kSplitMax tasks at
+// or above it (none when even the open top bin holds more), lists those
+// tasks and sets their flags, clears the split packets' completion counters
+// and every task's step count for this launch to fill in.
+constexpr int kSplitPlanThreads = 1024;
+static_assert(kSplitBins <= kSplitPlanThreads, "one thread per histogram bin");
+
+__global__ __launch_bounds__(kSplitPlanThreads) void k_split_plan(uint32_t* __restrict__ cost, uint64_t tasks,
+                                                                   uint32_t* __restrict__ flag,
+                                                                   uint32_t* __restrict__ list,
+                                                                   uint32_t* __restrict__ count,
+                                                                   uint32_t* __restrict__ sub_done,
+                                                                   uint32_t* __restrict__ unit_done, uint64_t units) {
+  __shared__ uint32_t hist[kSplitBins];
+  __shared__ uint32_t above_max;   // bins whose suffix count exceeds kSplitMax
+  __shared__ uint32_t listed;
+  const uint32_t tid = threadIdx.x;
+  for (uint32_t b = tid; b < (uint32_t)kSplitBins; b += kSplitPlanThreads) hist[b] = 0u;
+  if (tid == 0) {
+    above_max = 0u;
+    listed = 0u;
+  }
+  for (uint32_t i = tid; i < (uint32_t)kSplitMax; i += kSplitPlanThreads) sub_done[i] = 0u;
+  for (uint64_t u = tid; u < units; u += kSplitPlanThreads) unit_done[u] = 0u;
+  __syncthreads();
+#pragma unroll 4
+  for (uint64_t t = tid; t < tasks; t += kSplitPlanThreads)
+    atomicAdd(&hist[min(cost[t] / 8u, (uint32_t)kSplitBins - 1u)], 1u);
+  __syncthreads();
+  // suffix counts S[b] = tasks in bins b .. kSplitBins - 1 fall as b grows,
+  // so the bins with S[b] > kSplitMax are a prefix 0 .. above_max - 1
+  if (tid < (uint32_t)kSplitBins) {
+    uint32_t suffix = 0u;
+    for (uint32_t k = tid; k < (uint32_t)kSplitBins; ++k) suffix += hist[k];
+    if (suffix > (uint32_t)kSplitMax) atomicAdd(&above_max, 1u);
+  }
+  __syncthreads();
+  const uint32_t b = max(above_max, kSplitMinSteps / 8u);
+  const uint32_t threshold = b >= (uint32_t)kSplitBins ? 0xffffffffu : b * 8u;
+#pragma unroll 4
+  for (uint64_t t = tid; t < tasks; t += kSplitPlanThreads) {
+    uint32_t f = 0u;
+    if (cost[t] >= threshold) {
+      const uint32_t slot = atomicAdd(&listed, 1u);
+      if (slot < (uint32_t)kSplitMax) {
+        list[slot] = (uint32_t)t;
+        f = 1u;
+      }
+    }
+    flag[t] = f;
+    cost[t] = 0u;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    count[0] = min(listed, (uint32_t)kSplitMax);
+    count[1] = threshold;
+  }
 }
 
 __global__ void k_fill_u32(uint32_t* __restrict__ p, uint32_t n, uint32_t v) {
@@ -1196,20 +1299,35 @@ bool launch_render_units(const DeviceScene& scene, const UnitLaunch& U_in, hipSt
       U.chunk_partial && U.chunk_done && units * chunks <= 0xffffffffull - 65536ull &&
       units * chunks * 64u * per_chunk <= U.chunk_partial_cap && units <= U.chunk_done_cap) {
     const uint64_t tasks = units * chunks;
-    if (packet_tasks && tasks / (kTraceThreads / 64) < 0x7fffffffull) {
+    if (packet_tasks && tasks / (kTraceThreads / 64) < 0x7fffffffull - (uint64_t)kSplitMax * kSplitWays) {
       // wave packets: the non-persistent task kernel, one wave per chunk
-      hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((units + 255) / 256)), dim3(256), 0, stream, U.chunk_done,
-                         (uint32_t)units, 0u);
-      const dim3 grid((unsigned)((tasks + (kTraceThreads / 64) - 1) / (kTraceThreads / 64))), block(kTraceThreads);
+      UnitLaunch T = U;
+      const bool have_split = U.split_cost && U.split_flag && U.split_list && U.split_count && U.split_sub_done &&
+                              U.split_tasks == tasks;
+      if (have_split && U.split_plan) {
+        // the plan kernel also clears the units' completion counters
+        hipLaunchKernelGGL(k_split_plan, dim3(1), dim3(kSplitPlanThreads), 0, stream, U.split_cost, tasks,
+                           U.split_flag, U.split_list, U.split_count, U.split_sub_done, U.chunk_done, units);
+      } else {
+        hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((units + 255) / 256)), dim3(256), 0, stream, U.chunk_done,
+                           (uint32_t)units, 0u);
+        T.split_flag = nullptr;
+        if (have_split)  // no steps of these tasks yet: record them, split nothing
+          hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((tasks + 255) / 256)), dim3(256), 0, stream, U.split_cost,
+                             (uint32_t)tasks, 0u);
+        else
+          T.split_cost = nullptr;
+      }
+      const uint64_t waves = tasks + (T.split_flag ? (uint64_t)kSplitMax * kSplitWays : 0u);
+      const dim3 grid((unsigned)((waves + (kTraceThreads / 64) - 1) / (kTraceThreads / 64))), block(kTraceThreads);
       const bool scalar = U.traversal == kUnitPacketScalar;
-      if (U.samples && scalar)
-        hipLaunchKernelGGL((k_render_tasks<true, true>), grid, block, 0, stream, v, U, (int)chunks);
-      else if (U.samples)
-        hipLaunchKernelGGL((k_render_tasks<true, false>), grid, block, 0, stream, v, U, (int)chunks);
-      else if (scalar)
-        hipLaunchKernelGGL((k_render_tasks<false, true>), grid, block, 0, stream, v, U, (int)chunks);
-      else
-        hipLaunchKernelGGL((k_render_tasks<false, false>), grid, block, 0, stream, v, U, (int)chunks);
+      // the variant without the split's code keeps the registers of the plain task kernel
+      auto kernel = U.samples ? (scalar ? k_render_tasks<true, true, false> : k_render_tasks<true, false, false>)
+                              : (scalar ? k_render_tasks<false, true, false> : k_render_tasks<false, false, false>);
+      if (T.split_cost)
+        kernel = U.samples ? (scalar ? k_render_tasks<true, true, true> : k_render_tasks<true, false, true>)
+                           : (scalar ? k_render_tasks<false, true, true> : k_render_tasks<false, false, true>);
+      hipLaunchKernelGGL(kernel, grid, block, 0, stream, v, T, (int)chunks);
       g_last_kernel = "k_render_tasks";
       return check_launch("render_tasks", err);
     }

@@ -161,6 +161,8 @@ def configure_extensions(lib):
     lib.setRenderTuning.argtypes = [c_int, c_int]
     lib.getLastRenderKernel.restype = c_char_p
     lib.setSplitLaunches.argtypes = [c_bool]
+    lib.getCurrentEyeTaskSplits.restype = c_int
+    lib.setTaskSplit.argtypes = [c_int]
     lib.getRenderTuning.argtypes = [i32p]
     lib.getRenderTuning.restype = c_int
     lib.setBatchAutotune.argtypes = [c_int]

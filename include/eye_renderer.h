@@ -279,12 +279,25 @@ void setTraversalMode(int mode);
  * default: the packet renderer reduces samples in registers). */
 void setSampleRecording(bool enabled);
 /* Kernel of the last compound-eye render: "k_render_units", or for a split
- * launch (uniform counts above 64 samples and few units, where each unit's
- * 64-sample chunks run on different waves) "k_render_tasks" (wave packets)
- * or "k_render_chunks" (per-ray traversal). */
+ * launch (uniform counts and few units, where each unit's 64-sample chunks
+ * run on different waves) "k_render_tasks" (wave packets, any count) or
+ * "k_render_chunks" (per-ray traversal, above 32 samples). */
 const char* getLastRenderKernel(void);
 /* Allows split launches (default true); results are the same either way. */
 void setSplitLaunches(bool enabled);
+/* Long-task split of the wave-packet task kernel (an eye rendered frame after
+ * frame at a uniform rate): the tasks (64-sample chunks) that took the most
+ * packet steps in the previous render of the same eye are traced first, each
+ * as two packets of neighbouring-elevation rays.  mode 0: never; 1: always;
+ * 2 (default): per eye, by the measured trace times of a few renders with and
+ * without it, tried again every 1,000 renders and after any change of the
+ * rays.  Results are the same either way.  No counterpart in the reference
+ * (scheduling of this renderer). */
+void setTaskSplit(int mode);
+/* Tasks of the last renderFrame of the current eye traced split (0 when the
+ * split was off for that render or nothing was long enough).  No counterpart
+ * in the reference (diagnostic of this renderer's scheduling). */
+int getCurrentEyeTaskSplits(void);
 /* Launch shape of the persistent unit renderer: ommatidia taken from the work
  * queue per grab (0 = automatic, at most 64: a wave holds one grab's sample
  * counts in its lanes), and blocks of 256 threads (0 = every resident slot

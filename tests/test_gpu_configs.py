@@ -79,6 +79,82 @@ def test_c2_bvh_bit_exact(lib, config2):
     check_bvh(lib, config2["oracle"])
 
 
+@pytest.mark.parametrize("mode", [1, 2])
+def test_c2_long_task_split_bit_exact(lib, config2, mode):
+    """renderFrame of one bee-scale eye frame after frame at 128 spp: each
+    frame lists its longest tasks (with no history yet, the first ones of at
+    least the minimum length) and the next frame traces them first, as split
+    packets of neighbouring-elevation rays.  Every frame's colours equal the
+    oracle's, with sample recording off and on."""
+    lib.setTraversalMode(mode)
+    lib.setTaskSplit(1)
+    load(lib, config2["gltf"])
+    eye = compound_eyes(lib)[0]
+    er.setRenderSize(lib, 5500, 1)
+
+    def frame(spp, record):
+        lib.setSampleRecording(record)
+        assert lib.renderFrame() >= 0, er.last_error(lib)
+        assert lib.getLastRenderKernel() == b"k_render_tasks"
+        want = config2["oracle"].render(config2["eyes"][0], er.pose_np(lib), spp=spp,
+                                        eye=lib.getCurrentEyeIndex(),
+                                        frame=lib.getCurrentEyeFrameIndex() - 1, want_samples=record)
+        assert np.array_equal(er.colours_np(lib), want["colour_gpu_order"][0])
+        if record:
+            assert np.array_equal(er.samples_np(lib).view(np.uint32), want["samples"].view(np.uint32))
+        return lib.getCurrentEyeTaskSplits()
+
+    try:
+        lib.gotoCamera(eye)
+        lib.setCurrentEyeSamplesPerOmmatidium(128)
+        lib.setCameraPose(*[float(v) for v in config2["pose"]])
+        splits = [frame(128, record) for record in (False, False, False, True)]
+        assert splits[0] == 0 and min(splits[1:]) > 0, splits
+        # a change of rate starts over; at 96 spp the second chunk of each
+        # ommatidium has 32 rays, and its split packets trace what falls to
+        # them of those
+        lib.setCurrentEyeSamplesPerOmmatidium(96)
+        splits = [frame(96, False) for _ in range(3)]
+        assert splits[0] == 0 and min(splits[1:]) > 0, splits
+    finally:
+        lib.setTaskSplit(2)
+        lib.setSampleRecording(False)
+        lib.setTraversalMode(1)
+        er.setRenderSize(lib, 400, 400)
+
+
+def test_c2_task_split_trial(lib, config2):
+    """The default choice of the long-task split: renders 0-2 of an eye trace
+    without it, render 3 records the tasks' steps, renders 4 and 5 trace with
+    it, and later renders keep the faster way; every render's colours equal
+    the oracle's."""
+    lib.setTraversalMode(1)
+    lib.setTaskSplit(2)
+    load(lib, config2["gltf"])
+    eye = compound_eyes(lib)[0]
+    er.setRenderSize(lib, 5500, 1)
+    try:
+        lib.gotoCamera(eye)
+        lib.setCurrentEyeSamplesPerOmmatidium(128)
+        lib.setCameraPose(*[float(v) for v in config2["pose"]])
+        splits = []
+        for k in range(8):
+            assert lib.renderFrame() >= 0, er.last_error(lib)
+            splits.append(lib.getCurrentEyeTaskSplits())
+            want = config2["oracle"].render(config2["eyes"][0], er.pose_np(lib), spp=128,
+                                            eye=lib.getCurrentEyeIndex(),
+                                            frame=lib.getCurrentEyeFrameIndex() - 1)
+            assert np.array_equal(er.colours_np(lib), want["colour_gpu_order"][0])
+        assert splits[:4] == [0, 0, 0, 0] and min(splits[4:6]) > 0, splits
+        # renders 6 and 7 keep one way
+        assert (splits[6] > 0) == (splits[7] > 0), splits
+        lib.setTaskSplit(3)
+        assert er.last_error(lib)
+    finally:
+        lib.setTaskSplit(2)
+        er.setRenderSize(lib, 400, 400)
+
+
 @pytest.mark.parametrize("mode,kernel", [(1, b"k_render_tasks"), (2, b"k_render_tasks"), (0, b"k_render_chunks")])
 def test_c2_render_frame_bit_exact(lib, config2, mode, kernel):
     """Both eyes at 128 samples, recording off: the benchmark's renderFrame."""
