@@ -865,8 +865,12 @@ __global__ __launch_bounds__(kTraceThreads, 6) void k_render_tasks(SceneView S, 
   const int lane = threadIdx.x & 63;
   const uint64_t units = (uint64_t)U.pose_count * (uint64_t)U.omm_count;
   const uint64_t tasks = units * (uint64_t)chunks;
-  const uint64_t wv = (uint64_t)xcd_logical_block(blockIdx.x, gridDim.x) * (kTraceThreads / 64) +
-                      (uint64_t)(threadIdx.x >> 6);
+  // With vector node loads the task, unit and output indices go to scalar
+  // registers (the wave index read once as uniform), which keeps the split
+  // variant from spilling; the scalar-load traversal needs those registers
+  // for node words and keeps the indices in vector registers.
+  const uint32_t wave = kScalar ? (threadIdx.x >> 6) : (uint32_t)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const uint64_t wv = (uint64_t)xcd_logical_block(blockIdx.x, gridDim.x) * (kTraceThreads / 64) + (uint64_t)wave;
   // With a long-task split the first kSplitWays * kSplitMax waves trace the
   // listed tasks' split packets (wave w: part w % kSplitWays of listed task
   // w / kSplitWays), the rest every other task in order.
@@ -991,6 +995,7 @@ __global__ __launch_bounds__(64) void k_packet_guard_probe(SceneView S, const Ra
 // tasks and sets their flags, clears the split packets' completion counters
 // and every task's step count for this launch to fill in.
 constexpr int kSplitPlanThreads = 1024;
+constexpr int kSplitPlanPer = 16;   // step counts a thread holds at once
 static_assert(kSplitBins <= kSplitPlanThreads, "one thread per histogram bin");
 
 __global__ __launch_bounds__(kSplitPlanThreads) void k_split_plan(uint32_t* __restrict__ cost, uint64_t tasks,
@@ -1000,43 +1005,65 @@ __global__ __launch_bounds__(kSplitPlanThreads) void k_split_plan(uint32_t* __re
                                                                    uint32_t* __restrict__ sub_done,
                                                                    uint32_t* __restrict__ unit_done, uint64_t units) {
   __shared__ uint32_t hist[kSplitBins];
-  __shared__ uint32_t above_max;   // bins whose suffix count exceeds kSplitMax
   __shared__ uint32_t listed;
   const uint32_t tid = threadIdx.x;
   for (uint32_t b = tid; b < (uint32_t)kSplitBins; b += kSplitPlanThreads) hist[b] = 0u;
-  if (tid == 0) {
-    above_max = 0u;
-    listed = 0u;
-  }
+  if (tid == 0) listed = 0u;
   for (uint32_t i = tid; i < (uint32_t)kSplitMax; i += kSplitPlanThreads) sub_done[i] = 0u;
   for (uint64_t u = tid; u < units; u += kSplitPlanThreads) unit_done[u] = 0u;
   __syncthreads();
-#pragma unroll 4
-  for (uint64_t t = tid; t < tasks; t += kSplitPlanThreads)
-    atomicAdd(&hist[min(cost[t] / 8u, (uint32_t)kSplitBins - 1u)], 1u);
-  __syncthreads();
-  // suffix counts S[b] = tasks in bins b .. kSplitBins - 1 fall as b grows,
-  // so the bins with S[b] > kSplitMax are a prefix 0 .. above_max - 1
-  if (tid < (uint32_t)kSplitBins) {
-    uint32_t suffix = 0u;
-    for (uint32_t k = tid; k < (uint32_t)kSplitBins; ++k) suffix += hist[k];
-    if (suffix > (uint32_t)kSplitMax) atomicAdd(&above_max, 1u);
+  // a round's loads are all issued before its histogram updates; only tasks
+  // of at least kSplitMinSteps enter the histogram (shorter ones are never
+  // split, and the threshold search starts there)
+  const uint64_t per_round = (uint64_t)kSplitPlanThreads * kSplitPlanPer;
+  uint32_t c[kSplitPlanPer];
+  for (uint64_t base = 0; base < tasks; base += per_round) {
+#pragma unroll
+    for (int k = 0; k < kSplitPlanPer; ++k) {
+      const uint64_t t = base + (uint64_t)k * kSplitPlanThreads + tid;
+      c[k] = t < tasks ? cost[t] : 0u;
+    }
+#pragma unroll
+    for (int k = 0; k < kSplitPlanPer; ++k)
+      if (c[k] >= kSplitMinSteps) atomicAdd(&hist[min(c[k] / 8u, (uint32_t)kSplitBins - 1u)], 1u);
   }
   __syncthreads();
-  const uint32_t b = max(above_max, kSplitMinSteps / 8u);
+  // suffix counts S[b] = tasks in bins b .. kSplitBins - 1 fall as b grows,
+  // so the bins with S[b] > kSplitMax are a prefix of the bins
+  for (uint32_t off = 1; off < (uint32_t)kSplitBins; off <<= 1) {
+    const uint32_t v = tid + off < (uint32_t)kSplitBins ? hist[tid + off] : 0u;
+    __syncthreads();
+    if (tid < (uint32_t)kSplitBins) hist[tid] += v;
+    __syncthreads();
+  }
+  const uint32_t over =
+      (uint32_t)__syncthreads_count(tid < (uint32_t)kSplitBins && hist[tid] > (uint32_t)kSplitMax);
+  const uint32_t b = max(over, kSplitMinSteps / 8u);
   const uint32_t threshold = b >= (uint32_t)kSplitBins ? 0xffffffffu : b * 8u;
-#pragma unroll 4
-  for (uint64_t t = tid; t < tasks; t += kSplitPlanThreads) {
-    uint32_t f = 0u;
-    if (cost[t] >= threshold) {
-      const uint32_t slot = atomicAdd(&listed, 1u);
-      if (slot < (uint32_t)kSplitMax) {
-        list[slot] = (uint32_t)t;
-        f = 1u;
+  for (uint64_t base = 0; base < tasks; base += per_round) {
+    if (tasks > per_round) {   // one round: the step counts are still held
+#pragma unroll
+      for (int k = 0; k < kSplitPlanPer; ++k) {
+        const uint64_t t = base + (uint64_t)k * kSplitPlanThreads + tid;
+        c[k] = t < tasks ? cost[t] : 0u;
       }
     }
-    flag[t] = f;
-    cost[t] = 0u;
+#pragma unroll
+    for (int k = 0; k < kSplitPlanPer; ++k) {
+      const uint64_t t = base + (uint64_t)k * kSplitPlanThreads + tid;
+      if (t < tasks) {
+        uint32_t f = 0u;
+        if (c[k] >= threshold) {
+          const uint32_t slot = atomicAdd(&listed, 1u);
+          if (slot < (uint32_t)kSplitMax) {
+            list[slot] = (uint32_t)t;
+            f = 1u;
+          }
+        }
+        flag[t] = f;
+        cost[t] = 0u;
+      }
+    }
   }
   __syncthreads();
   if (tid == 0) {
