@@ -71,7 +71,7 @@ struct EyeState {
   size_t d_split_cap = 0;
   uint64_t split_tasks = 0;       // tasks the buffers are laid out for
   bool split_valid = false;       // the cost words hold the steps of a render of those tasks
-  uint32_t split_last = 0;        // tasks split by the last render
+  uint32_t split_listed[3] = {0, 0, 0};   // tasks the last render listed per tier
   uint64_t split_generation = 0;  // scene the costs were measured on
   // automatic choice: renders of the same tasks since the last trial began,
   // and the shortest trace time (ms) of the trial's renders without / with
@@ -497,7 +497,7 @@ double render_compound(EyeState& e, const Cam& c) {
     split_choice(e, &rec, &plan);
     if (rec) {
       const uint32_t* before = e.d_split;
-      ensure_device(&e.d_split, &e.d_split_cap, (size_t)(2 * tasks + 2 * kSplitMax + 2));
+      ensure_device(&e.d_split, &e.d_split_cap, (size_t)split_buffer_words(tasks));
       if (e.d_split != before) {   // new buffer: no steps recorded in it yet
         e.split_valid = false;
         plan = false;
@@ -505,8 +505,8 @@ double render_compound(EyeState& e, const Cam& c) {
       U.split_cost = e.d_split;
       U.split_flag = U.split_cost + tasks;
       U.split_list = U.split_flag + tasks;
-      U.split_count = U.split_list + kSplitMax;
-      U.split_sub_done = U.split_count + 2;
+      U.split_count = U.split_list + kSplitListMax;
+      U.split_sub_done = U.split_count + kSplitCountWords;
       U.split_tasks = tasks;
       U.split_plan = plan;
     }
@@ -515,11 +515,12 @@ double render_compound(EyeState& e, const Cam& c) {
     split_restart(e);
     throw std::runtime_error(err);
   }
-  e.split_last = 0;
+  e.split_listed[0] = e.split_listed[1] = e.split_listed[2] = 0;
   const bool tasks_kernel = std::strcmp(last_render_kernel(), "k_render_tasks") == 0;
   if (U.split_cost && tasks_kernel) {
     if (U.split_plan)
-      hip_check(hipMemcpyAsync(&e.split_last, U.split_count, sizeof(uint32_t), hipMemcpyDeviceToHost, s.stream),
+      hip_check(hipMemcpyAsync(e.split_listed, U.split_count, sizeof(e.split_listed), hipMemcpyDeviceToHost,
+                               s.stream),
                 "split count");
     e.split_valid = true;
   } else {
@@ -1671,7 +1672,7 @@ void setTaskSplit(int mode) {
 
 int getCurrentEyeTaskSplits(void) {
   EyeState* e = current_eye();
-  return e ? (int)std::min<uint32_t>(e->split_last, (uint32_t)kSplitMax) : 0;
+  return e ? (int)(e->split_listed[0] + e->split_listed[1]) : 0;
 }
 
 void setRenderTuning(int unitsPerGrab, int blocks) {

@@ -26,12 +26,26 @@ struct EyeLaunch {
   uint32_t frame = 0;
 };
 
-// Long-task split of the packet task kernel (see UnitLaunch::split_cost).
-constexpr int kSplitWays = 2;             // packets a split task is traced as             // packets a split task is traced as
-constexpr int kSplitMax = 1024;           // tasks split per launch at most            // tasks split per launch at most
-constexpr uint32_t kSplitMinSteps = 128;  // a task that took fewer packet steps is never split  // a task that took fewer packet steps is never split
-constexpr int kSplitBins = 256;           // threshold search: histogram bins of 8 steps
-static_assert(64 % kSplitWays == 0, "a split packet traces 64 / kSplitWays lanes");
+// Long-task split of the packet task kernel (see UnitLaunch::split_cost).  The
+// tasks listed from the last launch's packet steps are traced before every
+// other task, in three tiers by length: the longest as 4 packets of 16 rays,
+// the next as 2 packets of 32 rays, the next whole (started early).
+constexpr int kSplit4Max = 128;             // tier 0: tasks traced as 4 packets, at most
+constexpr int kSplit2Max = 896;             // tier 1: tasks traced as 2 packets, at most
+constexpr int kSplit1Max = 2048;            // tier 2: tasks started early, whole, at most
+constexpr uint32_t kSplitMinSteps = 128;    // tiers 0 and 1: a task of fewer packet steps is never split
+constexpr uint32_t kSplitEarlyMinSteps = 64;  // tier 2: nor started early
+constexpr int kSplitListMax = kSplit4Max + kSplit2Max + kSplit1Max;
+constexpr int kSplitSubMax = kSplit4Max + kSplit2Max;   // listed tasks traced as several packets
+constexpr int kSplitSubWaves = 4 * kSplit4Max + 2 * kSplit2Max + kSplit1Max;
+constexpr int kSplitCountWords = 8;         // listed per tier, then the tiers' thresholds in steps
+constexpr int kSplitBins = 256;             // threshold search: histogram bins of 8 steps
+// Device words of a split buffer for `tasks` tasks: cost and flag per task,
+// the tiers' lists, the counts and thresholds, a completion counter per
+// split task.
+constexpr uint64_t split_buffer_words(uint64_t tasks) {
+  return 2 * tasks + (uint64_t)kSplitListMax + kSplitCountWords + kSplitSubMax;
+}
 
 // Persistent ommatidium-unit renderer.  A unit is one ommatidium at one pose;
 // waves take units from a work queue (`units_per_grab` at a time), trace a
@@ -66,16 +80,16 @@ struct UnitLaunch {
   // Optional long-task split of the packet task kernel (an eye rendered frame
   // after frame, one pose, uniform counts).  The packet steps each task took
   // in the last launch (split_cost, per task) predict this launch's longest
-  // tasks: up to kSplitMax of them, at least kSplitMinSteps long, are traced
-  // as kSplitWays packets of 64 / kSplitWays rays of neighbouring elevation,
-  // started before every other task.  Results do not depend on it.  Device
-  // buffers: split_cost and split_flag (split_tasks entries each), split_list
-  // and split_sub_done (kSplitMax), split_count (2).  split_plan: split_cost
-  // holds the steps of a launch of the same tasks.
+  // tasks, which are listed (split_list, three tiers of kSplit4Max,
+  // kSplit2Max and kSplit1Max entries) and flagged (split_flag: tier + 1) and
+  // traced first, split into packets of rays of neighbouring elevation or
+  // whole.  Results do not depend on it.  split_count: kSplitCountWords;
+  // split_sub_done: kSplitSubMax.  split_plan: split_cost holds the steps of
+  // a launch of the same tasks.
   uint32_t* split_cost = nullptr;
   uint32_t* split_flag = nullptr;
   uint32_t* split_list = nullptr;
-  uint32_t* split_count = nullptr;      // [0] tasks split, [1] threshold in steps
+  uint32_t* split_count = nullptr;      // [0..2] tasks listed per tier, [3..5] the tiers' thresholds
   uint32_t* split_sub_done = nullptr;
   uint64_t split_tasks = 0;
   bool split_plan = false;

@@ -871,24 +871,44 @@ __global__ __launch_bounds__(kTraceThreads, 6) void k_render_tasks(SceneView S, 
   // for node words and keeps the indices in vector registers.
   const uint32_t wave = kScalar ? (threadIdx.x >> 6) : (uint32_t)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   const uint64_t wv = (uint64_t)xcd_logical_block(blockIdx.x, gridDim.x) * (kTraceThreads / 64) + (uint64_t)wave;
-  // With a long-task split the first kSplitWays * kSplitMax waves trace the
-  // listed tasks' split packets (wave w: part w % kSplitWays of listed task
-  // w / kSplitWays), the rest every other task in order.
+  // With a long-task split the first kSplitSubWaves waves trace the listed
+  // tasks: 4 waves per tier-0 task (wave w: part w % 4 of tier-0 task w / 4),
+  // then 2 per tier-1 task, then one per tier-2 task (whole); the rest every
+  // other task in order.
   const bool split = kSplit && U.split_flag != nullptr;
-  const uint64_t sub_waves = split ? (uint64_t)kSplitWays * (uint64_t)kSplitMax : 0u;
+  const uint64_t sub_waves = split ? (uint64_t)kSplitSubWaves : 0u;
   uint64_t t;
-  int part = -1;
-  uint32_t li = 0;
+  int part = -1, ways = 1;
+  uint32_t li = 0;   // completion counter of a split task
   if (wv < sub_waves) {
-    li = (uint32_t)(wv / (uint64_t)kSplitWays);
-    part = (int)(wv % (uint64_t)kSplitWays);
-    const uint32_t listed = (uint32_t)__builtin_amdgcn_readfirstlane((int)U.split_count[0]);
-    if (li >= listed || li >= (uint32_t)kSplitMax) return;  // whole wave
-    t = (uint32_t)__builtin_amdgcn_readfirstlane((int)U.split_list[li]);
+    constexpr uint32_t r0 = 4u * kSplit4Max, r1 = r0 + 2u * kSplit2Max;
+    const uint32_t w = (uint32_t)wv;
+    uint32_t tier, k, first;
+    if (w < r0) {
+      tier = 0u;
+      k = w >> 2;
+      part = (int)(w & 3u);
+      ways = 4;
+      first = 0u;
+    } else if (w < r1) {
+      tier = 1u;
+      k = (w - r0) >> 1;
+      part = (int)((w - r0) & 1u);
+      ways = 2;
+      first = kSplit4Max;
+    } else {
+      tier = 2u;
+      k = w - r1;
+      first = kSplit4Max + kSplit2Max;
+    }
+    const uint32_t listed = (uint32_t)__builtin_amdgcn_readfirstlane((int)U.split_count[tier]);
+    if (k >= listed) return;  // whole wave
+    t = (uint32_t)__builtin_amdgcn_readfirstlane((int)U.split_list[first + k]);
+    li = first + k;
   } else {
     t = wv - sub_waves;
     if (t >= tasks) return;  // whole wave
-    if (split && __builtin_amdgcn_readfirstlane((int)U.split_flag[t]) != 0) return;  // traced split
+    if (split && __builtin_amdgcn_readfirstlane((int)U.split_flag[t]) != 0) return;  // traced listed
   }
   const uint32_t n = (uint32_t)U.spp;
   const float inv = 1.0f / (float)n;
@@ -912,8 +932,8 @@ __global__ __launch_bounds__(kTraceThreads, 6) void k_render_tasks(SceneView S, 
     ray.tmax = 1.0f;
   }
   const RayPre rp = precompute_ray(ray);
-  // A split packet traces the lanes whose rays rank part * 64 / kSplitWays ..
-  // (part + 1) * 64 / kSplitWays - 1 by elevation (the direction's y, ties by
+  // A split packet traces the lanes whose rays rank part * 64 / ways ..
+  // (part + 1) * 64 / ways - 1 by elevation (the direction's y, ties by
   // lane): rays of neighbouring elevation fan out less than the whole chunk.
   bool mine = active;
   if (part >= 0) {
@@ -922,7 +942,7 @@ __global__ __launch_bounds__(kTraceThreads, 6) void k_render_tasks(SceneView S, 
     const uint32_t sorted = wave_sort_desc(key | (uint32_t)lane);
     // sorted position q holds lane (sorted & 63): send q back to that lane
     const int rank = __builtin_amdgcn_ds_permute((int)((sorted & 63u) * 4u), lane);
-    mine = active && rank / (64 / kSplitWays) == part;
+    mine = active && (rank >> (ways == 4 ? 4 : 5)) == part;
   }
   Hit h;
   uint32_t w_steps = 0, w_tris = 0;
@@ -948,7 +968,7 @@ __global__ __launch_bounds__(kTraceThreads, 6) void k_render_tasks(SceneView S, 
     uint32_t done = 0;
     if (lane == 0) done = atomicAdd(U.split_sub_done + li, 1u);
     done = __builtin_amdgcn_readfirstlane(__shfl(done, 0));
-    if (done != (uint32_t)kSplitWays - 1u) return;  // whole wave
+    if (done != (uint32_t)ways - 1u) return;  // whole wave
     __threadfence();
   }
   uint32_t prev = 0;
@@ -989,14 +1009,21 @@ __global__ __launch_bounds__(64) void k_packet_guard_probe(SceneView S, const Ra
 
 // Long-task split plan, one workgroup launched before the task kernel: it
 // clears the units' completion counters, takes a histogram of the last
-// launch's per-task packet steps (bins of 8 steps, the last one open), picks
-// the smallest step count T >= kSplitMinSteps with at most kSplitMax tasks at
-// or above it (none when even the open top bin holds more), lists those
-// tasks and sets their flags, clears the split packets' completion counters
-// and every task's step count for this launch to fill in.
+// launch's per-task packet steps (bins of 8 steps, the last one open), and
+// picks each tier's threshold: the smallest multiple of 8 steps at or above
+// the tier's minimum with at most the tiers' combined room (tier 0: 128
+// tasks, tiers 0-1: 1,024, tiers 0-2: 3,072) at or above it (none when even
+// the open top bin holds more).  It lists the tasks by tier and flags them,
+// and clears the split packets' completion counters and every task's step
+// count for this launch to fill in.
 constexpr int kSplitPlanThreads = 1024;
 constexpr int kSplitPlanPer = 16;   // step counts a thread holds at once
 static_assert(kSplitBins <= kSplitPlanThreads, "one thread per histogram bin");
+
+__device__ __forceinline__ uint32_t split_threshold(uint32_t over, uint32_t min_steps) {
+  const uint32_t b = max(over, min_steps / 8u);
+  return b >= (uint32_t)kSplitBins ? 0xffffffffu : b * 8u;
+}
 
 __global__ __launch_bounds__(kSplitPlanThreads) void k_split_plan(uint32_t* __restrict__ cost, uint64_t tasks,
                                                                    uint32_t* __restrict__ flag,
@@ -1005,16 +1032,16 @@ __global__ __launch_bounds__(kSplitPlanThreads) void k_split_plan(uint32_t* __re
                                                                    uint32_t* __restrict__ sub_done,
                                                                    uint32_t* __restrict__ unit_done, uint64_t units) {
   __shared__ uint32_t hist[kSplitBins];
-  __shared__ uint32_t listed;
+  __shared__ uint32_t listed[3];
   const uint32_t tid = threadIdx.x;
   for (uint32_t b = tid; b < (uint32_t)kSplitBins; b += kSplitPlanThreads) hist[b] = 0u;
-  if (tid == 0) listed = 0u;
-  for (uint32_t i = tid; i < (uint32_t)kSplitMax; i += kSplitPlanThreads) sub_done[i] = 0u;
+  if (tid < 3u) listed[tid] = 0u;
+  for (uint32_t i = tid; i < (uint32_t)kSplitSubMax; i += kSplitPlanThreads) sub_done[i] = 0u;
   for (uint64_t u = tid; u < units; u += kSplitPlanThreads) unit_done[u] = 0u;
   __syncthreads();
   // a round's loads are all issued before its histogram updates; only tasks
-  // of at least kSplitMinSteps enter the histogram (shorter ones are never
-  // split, and the threshold search starts there)
+  // of at least kSplitEarlyMinSteps enter the histogram (shorter ones are
+  // never listed, and the threshold searches start there or above)
   const uint64_t per_round = (uint64_t)kSplitPlanThreads * kSplitPlanPer;
   uint32_t c[kSplitPlanPer];
   for (uint64_t base = 0; base < tasks; base += per_round) {
@@ -1025,21 +1052,24 @@ __global__ __launch_bounds__(kSplitPlanThreads) void k_split_plan(uint32_t* __re
     }
 #pragma unroll
     for (int k = 0; k < kSplitPlanPer; ++k)
-      if (c[k] >= kSplitMinSteps) atomicAdd(&hist[min(c[k] / 8u, (uint32_t)kSplitBins - 1u)], 1u);
+      if (c[k] >= kSplitEarlyMinSteps) atomicAdd(&hist[min(c[k] / 8u, (uint32_t)kSplitBins - 1u)], 1u);
   }
   __syncthreads();
   // suffix counts S[b] = tasks in bins b .. kSplitBins - 1 fall as b grows,
-  // so the bins with S[b] > kSplitMax are a prefix of the bins
+  // so the bins with S[b] above a room are a prefix of the bins
   for (uint32_t off = 1; off < (uint32_t)kSplitBins; off <<= 1) {
     const uint32_t v = tid + off < (uint32_t)kSplitBins ? hist[tid + off] : 0u;
     __syncthreads();
     if (tid < (uint32_t)kSplitBins) hist[tid] += v;
     __syncthreads();
   }
-  const uint32_t over =
-      (uint32_t)__syncthreads_count(tid < (uint32_t)kSplitBins && hist[tid] > (uint32_t)kSplitMax);
-  const uint32_t b = max(over, kSplitMinSteps / 8u);
-  const uint32_t threshold = b >= (uint32_t)kSplitBins ? 0xffffffffu : b * 8u;
+  const bool bin = tid < (uint32_t)kSplitBins;
+  const uint32_t over0 = (uint32_t)__syncthreads_count(bin && hist[tid] > (uint32_t)kSplit4Max);
+  const uint32_t over1 = (uint32_t)__syncthreads_count(bin && hist[tid] > (uint32_t)(kSplit4Max + kSplit2Max));
+  const uint32_t over2 = (uint32_t)__syncthreads_count(bin && hist[tid] > (uint32_t)kSplitListMax);
+  const uint32_t th0 = split_threshold(over0, kSplitMinSteps);
+  const uint32_t th1 = split_threshold(over1, kSplitMinSteps);
+  const uint32_t th2 = split_threshold(over2, kSplitEarlyMinSteps);
   for (uint64_t base = 0; base < tasks; base += per_round) {
     if (tasks > per_round) {   // one round: the step counts are still held
 #pragma unroll
@@ -1053,11 +1083,14 @@ __global__ __launch_bounds__(kSplitPlanThreads) void k_split_plan(uint32_t* __re
       const uint64_t t = base + (uint64_t)k * kSplitPlanThreads + tid;
       if (t < tasks) {
         uint32_t f = 0u;
-        if (c[k] >= threshold) {
-          const uint32_t slot = atomicAdd(&listed, 1u);
-          if (slot < (uint32_t)kSplitMax) {
-            list[slot] = (uint32_t)t;
-            f = 1u;
+        const uint32_t tier = c[k] >= th0 ? 0u : c[k] >= th1 ? 1u : c[k] >= th2 ? 2u : 3u;
+        if (tier < 3u) {
+          const uint32_t room = tier == 0u ? kSplit4Max : tier == 1u ? kSplit2Max : kSplit1Max;
+          const uint32_t first = tier == 0u ? 0u : tier == 1u ? kSplit4Max : kSplit4Max + kSplit2Max;
+          const uint32_t slot = atomicAdd(&listed[tier], 1u);
+          if (slot < room) {
+            list[first + slot] = (uint32_t)t;
+            f = tier + 1u;
           }
         }
         flag[t] = f;
@@ -1067,8 +1100,12 @@ __global__ __launch_bounds__(kSplitPlanThreads) void k_split_plan(uint32_t* __re
   }
   __syncthreads();
   if (tid == 0) {
-    count[0] = min(listed, (uint32_t)kSplitMax);
-    count[1] = threshold;
+    count[0] = min(listed[0], (uint32_t)kSplit4Max);
+    count[1] = min(listed[1], (uint32_t)kSplit2Max);
+    count[2] = min(listed[2], (uint32_t)kSplit1Max);
+    count[3] = th0;
+    count[4] = th1;
+    count[5] = th2;
   }
 }
 
@@ -1326,7 +1363,7 @@ bool launch_render_units(const DeviceScene& scene, const UnitLaunch& U_in, hipSt
       U.chunk_partial && U.chunk_done && units * chunks <= 0xffffffffull - 65536ull &&
       units * chunks * 64u * per_chunk <= U.chunk_partial_cap && units <= U.chunk_done_cap) {
     const uint64_t tasks = units * chunks;
-    if (packet_tasks && tasks / (kTraceThreads / 64) < 0x7fffffffull - (uint64_t)kSplitMax * kSplitWays) {
+    if (packet_tasks && tasks / (kTraceThreads / 64) < 0x7fffffffull - (uint64_t)kSplitSubWaves) {
       // wave packets: the non-persistent task kernel, one wave per chunk
       UnitLaunch T = U;
       const bool have_split = U.split_cost && U.split_flag && U.split_list && U.split_count && U.split_sub_done &&
@@ -1345,7 +1382,7 @@ bool launch_render_units(const DeviceScene& scene, const UnitLaunch& U_in, hipSt
         else
           T.split_cost = nullptr;
       }
-      const uint64_t waves = tasks + (T.split_flag ? (uint64_t)kSplitMax * kSplitWays : 0u);
+      const uint64_t waves = tasks + (T.split_flag ? (uint64_t)kSplitSubWaves : 0u);
       const dim3 grid((unsigned)((waves + (kTraceThreads / 64) - 1) / (kTraceThreads / 64))), block(kTraceThreads);
       const bool scalar = U.traversal == kUnitPacketScalar;
       // the variant without the split's code keeps the registers of the plain task kernel

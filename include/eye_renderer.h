@@ -287,15 +287,16 @@ const char* getLastRenderKernel(void);
 void setSplitLaunches(bool enabled);
 /* Long-task split of the wave-packet task kernel (an eye rendered frame after
  * frame at a uniform rate): the tasks (64-sample chunks) that took the most
- * packet steps in the previous render of the same eye are traced first, each
- * as two packets of neighbouring-elevation rays.  mode 0: never; 1: always;
+ * packet steps in the previous render of the same eye are traced first, the
+ * longest as four or two packets of neighbouring-elevation rays, the next
+ * whole.  mode 0: never; 1: always;
  * 2 (default): per eye, by the measured trace times of a few renders with and
  * without it, tried again every 1,000 renders and after any change of the
  * rays.  Results are the same either way.  No counterpart in the reference
  * (scheduling of this renderer). */
 void setTaskSplit(int mode);
-/* Tasks of the last renderFrame of the current eye traced split (0 when the
- * split was off for that render or nothing was long enough).  No counterpart
+/* Tasks of the last renderFrame of the current eye traced as several packets
+ * (0 when the split was off for that render or nothing was long enough).  No counterpart
  * in the reference (diagnostic of this renderer's scheduling). */
 int getCurrentEyeTaskSplits(void);
 /* Launch shape of the persistent unit renderer: ommatidia taken from the work

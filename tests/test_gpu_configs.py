@@ -82,10 +82,10 @@ def test_c2_bvh_bit_exact(lib, config2):
 @pytest.mark.parametrize("mode", [1, 2])
 def test_c2_long_task_split_bit_exact(lib, config2, mode):
     """renderFrame of one bee-scale eye frame after frame at 128 spp: each
-    frame lists its longest tasks (with no history yet, the first ones of at
-    least the minimum length) and the next frame traces them first, as split
-    packets of neighbouring-elevation rays.  Every frame's colours equal the
-    oracle's, with sample recording off and on."""
+    frame lists its longest tasks and the next frame traces them first, the
+    longest as four or two packets of neighbouring-elevation rays, the next
+    whole.  Every frame's colours equal the oracle's, with sample recording
+    off and on."""
     lib.setTraversalMode(mode)
     lib.setTaskSplit(1)
     load(lib, config2["gltf"])
