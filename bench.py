@@ -831,9 +831,9 @@ def main() -> int:
     ap.add_argument("--plain-only", action="store_true",
                     help="run only the bee-scale renderFrame workload (profiler runs of k_render_tasks)")
     ap.add_argument("--no-split", action="store_true", help="disable split launches (chunks of a unit on several waves)")
-    ap.add_argument("--trace-mode", type=int, default=1, choices=(0, 1, 2),
-                    help="renderFrame traversal: 0 per-ray, 1 wave packets (default), 2 wave packets with scalar "
-                         "node loads")
+    ap.add_argument("--trace-mode", type=int, default=2, choices=(0, 1, 2),
+                    help="renderFrame traversal: 0 per-ray, 1 wave packets, 2 wave packets with scalar node loads "
+                         "(default)")
     ap.add_argument("--copy-probe", type=float, default=0.0,
                     help="also run a plain copy kernel of this many MiB (profiler bandwidth reference)")
     args = ap.parse_args()

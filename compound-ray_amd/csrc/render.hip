@@ -1241,7 +1241,7 @@ size_t stack_bytes(const DeviceScene& s) {
 
 }  // namespace
 
-static int g_trace_mode = kTracePacket;
+static int g_trace_mode = kTracePacketScalar;
 static const char* g_last_kernel = "";
 static bool g_split = true;
 

@@ -272,8 +272,9 @@ int setSceneLeafSize(int leafSize);
  * a later render can add samples independent of an earlier one. */
 void setCurrentEyeSampleBase(uint32_t base);
 /* Traversal strategy of eye rays: 0 = one ray per lane (own LDS stack),
- * 1 = wave packets (default), 2 = wave packets reading nodes and triangles
- * through the scalar unit.  Results are identical; only speed differs. */
+ * 1 = wave packets, 2 = wave packets reading nodes and triangles through the
+ * scalar unit (default).  Results are identical; only speed differs.
+ * autotuneRenderBatch sets it to the tuned winner. */
 void setTraversalMode(int mode);
 /* Keep the per-sample records of renderFrame for getCurrentEyeSamples (off by
  * default: the packet renderer reduces samples in registers). */

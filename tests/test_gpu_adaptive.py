@@ -70,7 +70,7 @@ def test_adaptive_independent_of_launch_shape(lib, config1, mode, grab, blocks):
         _, rgb1, counts1, total1 = er.render_adaptive_np(lib, poses6, error_target=0.02, max_spp=96)
     finally:
         lib.setRenderTuning(0, 0)
-        lib.setTraversalMode(1)
+        lib.setTraversalMode(2)
     assert total1 == total0
     assert np.array_equal(counts1, counts0)
     assert np.array_equal(rgb1, rgb0)

@@ -119,7 +119,7 @@ def test_c2_long_task_split_bit_exact(lib, config2, mode):
     finally:
         lib.setTaskSplit(2)
         lib.setSampleRecording(False)
-        lib.setTraversalMode(1)
+        lib.setTraversalMode(2)
         er.setRenderSize(lib, 400, 400)
 
 
@@ -152,6 +152,7 @@ def test_c2_task_split_trial(lib, config2):
         assert er.last_error(lib)
     finally:
         lib.setTaskSplit(2)
+        lib.setTraversalMode(2)
         er.setRenderSize(lib, 400, 400)
 
 
@@ -183,7 +184,7 @@ def test_c2_render_frame_bit_exact(lib, config2, mode, kernel):
                 assert st["nodes_visited"] == want["nodes_visited"]
                 assert st["tris_tested"] == want["tris_tested"]
     finally:
-        lib.setTraversalMode(1)
+        lib.setTraversalMode(2)
         er.setRenderSize(lib, 400, 400)
 
 
@@ -227,7 +228,7 @@ def test_c3_batch_autotuned_graph_bit_exact(lib, tmp_path):
             assert np.array_equal(rgb, want)
     finally:
         lib.setRenderTuning(0, 0)
-        lib.setTraversalMode(1)
+        lib.setTraversalMode(2)
         er.set_sample_counts(lib, None)
 
 

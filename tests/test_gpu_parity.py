@@ -124,7 +124,7 @@ def test_traversal_modes_bit_exact(lib, config1, oracle_c1, mode, spp):
         assert np.array_equal(er.samples_np(lib).view(np.uint32), ref["samples"].view(np.uint32))
         assert np.array_equal(er.colours_np(lib), ref["colour_gpu_order"][0])
     finally:
-        lib.setTraversalMode(1)
+        lib.setTraversalMode(2)
 
 
 @pytest.mark.parametrize("mode", [0, 1])
@@ -153,7 +153,7 @@ def test_split_launch_routing(lib, config1, oracle_c1, mode, spp):
             assert np.array_equal(er.colours_np(lib), ref["colour_gpu_order"][0])
     finally:
         lib.setSplitLaunches(True)
-        lib.setTraversalMode(1)
+        lib.setTraversalMode(2)
         lib.setSampleRecording(True)
 
 
@@ -371,7 +371,7 @@ def test_autotuned_batch_matches_oracle(lib, config1, oracle_c1, tmp_path):
         lib.setBatchAutotune(0)
         lib.setAutotuneCacheFile(None)
         lib.setRenderTuning(0, 0)
-        lib.setTraversalMode(1)
+        lib.setTraversalMode(2)
 
 
 def test_spp_edge_values(lib, config1, oracle_c1):
