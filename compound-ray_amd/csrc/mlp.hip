@@ -2,7 +2,8 @@
 // points).  Activations are kept transposed (features x samples) so that the
 // accumulator layout of one v_mfma_f32_16x16x16bf16 is directly the B operand
 // of the next layer: a wave pushes 16 samples through all four layers without
-// moving data between lanes.  Weight gradients need the samples as the
+// moving data between lanes.  The 64-input products pair two such blocks into
+// one gfx950 v_mfma_f32_16x16x32_bf16.  Weight gradients need the samples as the
 // reduction dimension, which one LDS transpose per layer provides.
 #include "mlp.h"
 
@@ -45,6 +46,18 @@ __device__ inline float bf2f(uint16_t b) { return __uint_as_float((uint32_t)b <<
 __device__ inline float bf2f_s(short b) { return bf2f((uint16_t)b); }
 
 __device__ inline f4 mfma(s4 a, s4 b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0); }
+
+// Two K = 16 products as one gfx950 K = 32 MFMA.  Lane group g = l/16 gives
+// the instruction k slots 8g .. 8g + 7; here slots 8g .. 8g + 3 hold the
+// first block's k = 4g .. 4g + 3 and slots 8g + 4 .. 8g + 7 the second's.  The
+// sum runs over every slot, so A and B only have to share that order.
+typedef short s8 __attribute__((ext_vector_type(8)));
+typedef __bf16 b8 __attribute__((ext_vector_type(8)));
+__device__ inline f4 mfma2(s4 a0, s4 a1, s4 b0, s4 b1, f4 c) {
+  const s8 a = __builtin_shufflevector(a0, a1, 0, 1, 2, 3, 4, 5, 6, 7);
+  const s8 b = __builtin_shufflevector(b0, b1, 0, 1, 2, 3, 4, 5, 6, 7);
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(b8, a), __builtin_bit_cast(b8, b), c, 0, 0, 0);
+}
 
 __device__ inline void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -108,8 +121,15 @@ __device__ inline void hidden_layer(const uint16_t* w, const float* b, const s4*
 #pragma unroll
   for (int ob = 0; ob < 4; ++ob) {
     f4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+    const uint16_t* row = w + kOff + (ob * 16 + l16) * kIn + 4 * q;
+    if constexpr (kIn % 32 == 0) {
 #pragma unroll
-    for (int kb = 0; kb < kIn / 16; ++kb) acc = mfma(lds4(w + kOff + (ob * 16 + l16) * kIn + kb * 16 + 4 * q), in[kb], acc);
+      for (int kb = 0; kb < kIn / 16; kb += 2)
+        acc = mfma2(lds4(row + kb * 16), lds4(row + kb * 16 + 16), in[kb], in[kb + 1], acc);
+    } else {
+#pragma unroll
+      for (int kb = 0; kb < kIn / 16; ++kb) acc = mfma(lds4(row + kb * 16), in[kb], acc);
+    }
     s4 h;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
@@ -195,8 +215,9 @@ __device__ inline void back_layer(const uint16_t* wn, const s4* dzb, const s4* h
 #pragma unroll
   for (int ib = 0; ib < 4; ++ib) {
     f4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+    const uint16_t* row = wn + (ib * 16 + l16) * 64 + 4 * q;
 #pragma unroll
-    for (int kb = 0; kb < 4; ++kb) acc = mfma(lds4(wn + (ib * 16 + l16) * 64 + kb * 16 + 4 * q), dzb[kb], acc);
+    for (int kb = 0; kb < 4; kb += 2) acc = mfma2(lds4(row + kb * 16), lds4(row + kb * 16 + 16), dzb[kb], dzb[kb + 1], acc);
     s4 o;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
