@@ -156,6 +156,38 @@ def test_c2_task_split_trial(lib, config2):
         er.setRenderSize(lib, 400, 400)
 
 
+def test_c2_task_split_two_eyes_alternating(lib, config2):
+    """Both bee-scale eyes rendered in turn, as the benchmark does, with the
+    split always on: each eye keeps its own step counts and lists (the other
+    eye's render in between changes neither), and every render of both eyes
+    equals the oracle."""
+    lib.setTraversalMode(2)
+    lib.setTaskSplit(1)
+    load(lib, config2["gltf"])
+    eyes = compound_eyes(lib)
+    er.setRenderSize(lib, 5500, 1)
+    try:
+        for i in eyes:
+            lib.gotoCamera(i)
+            lib.setCurrentEyeSamplesPerOmmatidium(128)
+            lib.setCameraPose(*[float(v) for v in config2["pose"]])
+        splits = {i: [] for i in eyes}
+        for k in range(3):
+            for n, i in enumerate(eyes):
+                lib.gotoCamera(i)
+                assert lib.renderFrame() >= 0, er.last_error(lib)
+                splits[i].append(lib.getCurrentEyeTaskSplits())
+                want = config2["oracle"].render(config2["eyes"][n], er.pose_np(lib), spp=128,
+                                                eye=lib.getCurrentEyeIndex(),
+                                                frame=lib.getCurrentEyeFrameIndex() - 1)
+                assert np.array_equal(er.colours_np(lib), want["colour_gpu_order"][0]), (k, i)
+        for i in eyes:
+            assert splits[i][0] == 0 and min(splits[i][1:]) > 0, splits
+    finally:
+        lib.setTaskSplit(2)
+        er.setRenderSize(lib, 400, 400)
+
+
 @pytest.mark.parametrize("mode,kernel", [(1, b"k_render_tasks"), (2, b"k_render_tasks"), (0, b"k_render_chunks")])
 def test_c2_render_frame_bit_exact(lib, config2, mode, kernel):
     """Both eyes at 128 samples, recording off: the benchmark's renderFrame."""
