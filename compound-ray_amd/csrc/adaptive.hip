@@ -305,7 +305,10 @@ bool AdaptiveRenderer::render(const DeviceScene& scene, UnitLaunch U, const std:
   A.queue = d_queue_;
   // Grab size chosen per pass (0: automatic).  A tuned size comes from uniform
   // batches; a grab of few pilot units would leave most of the wave's lanes idle.
+  // Pilot grabs of at most 8 units (one 64-lane round of 8-sample units):
+  // on the 256-eye batch 9.6 -> 8.9 ms against 16, the bee eye unchanged.
   A.units_per_grab = 0;
+  A.auto_grab_max = 8;
   if (!launch_render_units(scene, A, stream, err)) return false;
   (void)hipEventRecord(ev_stage_[0], stream);
   const unsigned gb = (unsigned)((units + kThreads - 1) / kThreads);
@@ -335,9 +338,11 @@ bool AdaptiveRenderer::render(const DeviceScene& scene, UnitLaunch U, const std:
   B.sample_base = (uint32_t)cfg.pilot;
   B.colours = d_col_b_;
   B.stats = nullptr;
-  // Top-up grabs of at most 8 units: a wave's rounds then mix the cones of
-  // fewer, closer ommatidia (measured 1-2 % faster than 16 on the bee eye).
-  B.auto_grab_max = 8;
+  // Top-up grabs of at most 6 units: a wave's rounds then mix the cones of
+  // fewer, closer ommatidia.  Against 8: the 256-eye batch's top-up 13.2 ->
+  // 12.6 ms, the bee eye unchanged; 4 and 2 are slower on the bee eye, 16
+  // slower on both.
+  B.auto_grab_max = 6;
   if (!launch_render_units(scene, B, stream, err)) return false;
   (void)hipEventRecord(ev_stage_[2], stream);
   hipLaunchKernelGGL(k_combine, dim3(gb), dim3(kThreads), 0, stream, d_col_a_, d_col_b_, d_extra_, cfg.pilot,
@@ -418,6 +423,7 @@ bool AdaptiveRenderer::training_set(const DeviceScene& scene, UnitLaunch U, cons
   A.samples = nullptr;
   A.queue = d_queue_;
   A.units_per_grab = 0;  // automatic, as in render()
+  A.auto_grab_max = 8;
   if (!launch_render_units(scene, A, stream, err)) return false;
   const unsigned gb = (unsigned)((units + kThreads - 1) / kThreads);
   hipLaunchKernelGGL(k_features, dim3(gb), dim3(kThreads), 0, stream, U.omms, d_poses_, U.omm_count, (uint64_t)units,
