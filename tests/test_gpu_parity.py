@@ -598,3 +598,29 @@ def test_packet_counts(lib, config1, oracle_c1):
     assert np.array_equal(er.colours_np(lib), ref["colour_gpu_order"][0])
     assert 0 < pk["nodes"] and ref["nodes_visited"] <= 64 * pk["nodes"]
     assert 0 < pk["tris"] and ref["tris_tested"] <= 64 * pk["tris"]
+
+
+def test_convergence_against_fp64_oracle(lib, config1, oracle_c1):
+    """Converged GPU colours against the FP64-accumulated 4,096-sample oracle:
+    the GPU's own 4,096-sample render (FP32 accumulation, the same samples)
+    stays within 1e-4 of it, and renders of independent samples approach it
+    as N^-1/2, within 1 % RMS from 1,024 samples on."""
+    omm = config1["eyes"][0][:120]
+    setup_eye(lib, config1, omm=omm, spp=4096, record=False)
+    try:
+        ref = oracle_c1.render(omm, er.pose_np(lib), spp=4096, want_f64=True, eye=lib.getCurrentEyeIndex(),
+                               frame=7)["colour_f64"][0]
+        lib.setCurrentEyeFrameIndex(7)
+        assert lib.renderFrame() >= 0, er.last_error(lib)
+        same = er.colours_np(lib).astype(np.float64)
+        assert float(np.max(np.abs(same - ref))) < 1e-4
+        errs = {}
+        for spp in (64, 256, 1024):
+            lib.setCurrentEyeSamplesPerOmmatidium(spp)
+            lib.setCurrentEyeFrameIndex(100 + spp)   # samples independent of the reference's
+            assert lib.renderFrame() >= 0, er.last_error(lib)
+            errs[spp] = float(np.sqrt(np.mean((er.colours_np(lib).astype(np.float64) - ref) ** 2)))
+        assert errs[1024] < 0.01, errs
+        assert 1.4 < errs[64] / errs[256] < 2.9 and 1.4 < errs[256] / errs[1024] < 2.9, errs
+    finally:
+        lib.setSampleRecording(True)
