@@ -156,6 +156,38 @@ def test_c2_task_split_trial(lib, config2):
         er.setRenderSize(lib, 400, 400)
 
 
+def test_c2_task_split_many_tasks(lib, config2):
+    """More tasks than the split plan holds in one round (a thread keeps 16
+    step counts, 16,384 per round): 300 of the bee eye's ommatidia at 4,096
+    samples are 19,200 tasks.  With the split on, every frame equals the
+    oracle and the later frames list long tasks."""
+    lib.setTraversalMode(2)
+    lib.setTaskSplit(1)
+    load(lib, config2["gltf"])
+    eye = compound_eyes(lib)[0]
+    omm = np.ascontiguousarray(config2["eyes"][0][::18][:300])
+    assert omm.shape[0] == 300
+    er.setRenderSize(lib, 300, 1)
+    try:
+        lib.gotoCamera(eye)
+        er.set_ommatidia_np(lib, omm)
+        lib.setCurrentEyeSamplesPerOmmatidium(4096)
+        lib.setCameraPose(*[float(v) for v in config2["pose"]])
+        splits = []
+        for k in range(3):
+            assert lib.renderFrame() >= 0, er.last_error(lib)
+            assert lib.getLastRenderKernel() == b"k_render_tasks"
+            splits.append(lib.getCurrentEyeTaskSplits())
+            want = config2["oracle"].render(omm, er.pose_np(lib), spp=4096, eye=lib.getCurrentEyeIndex(),
+                                            frame=lib.getCurrentEyeFrameIndex() - 1)
+            assert np.array_equal(er.colours_np(lib), want["colour_gpu_order"][0]), k
+        assert splits[0] == 0 and min(splits[1:]) > 0, splits
+    finally:
+        lib.setTaskSplit(2)
+        er.set_ommatidia_np(lib, config2["eyes"][0])
+        er.setRenderSize(lib, 400, 400)
+
+
 def test_c2_task_split_two_eyes_alternating(lib, config2):
     """Both bee-scale eyes rendered in turn, as the benchmark does, with the
     split always on: each eye keeps its own step counts and lists (the other
