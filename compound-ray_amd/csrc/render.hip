@@ -550,7 +550,9 @@ __device__ __forceinline__ int pow2_at_least(uint32_t n) {
 // unit bookkeeping to scratch and reloads it outside the traversal loop.  The
 // per-ray variant's LDS stack columns allow few resident blocks anyway; four
 // waves (128 VGPRs) keep its traversal loop free of scratch.
-template <bool kStats, bool kSamples, int kTrav>
+// kOnePass: every unit has at most 64 samples (a uniform count), so a round's
+// lanes trace once; the sums then need no registers across the traversal.
+template <bool kStats, bool kSamples, int kTrav, bool kOnePass = false>
 __global__ __launch_bounds__(kTraceThreads, kTrav == kUnitPerRay ? 4 : 6) void k_render_units(SceneView S,
                                                                                              UnitLaunch U) {
   // LDS (per-ray traversal only): one stack column per lane, laid out
@@ -695,6 +697,7 @@ __global__ __launch_bounds__(kTraceThreads, kTrav == kUnitPerRay ? 4 : 6) void k
             sh += hit ? 1.0f : 0.0f;
           }
         }
+        if (kOnePass) break;
       }
       const int seg_len = 1 << ((desc >> 8) & 7u);
       r = segment_sum(r, seg_len);
@@ -1249,7 +1252,13 @@ template <int kTrav>
 static void launch_units_variant(const UnitLaunch& U, const SceneView& v, int blocks, size_t lds,
                                  hipStream_t stream) {
   const dim3 grid(blocks), block(kTraceThreads);
-  if (U.stats && U.samples)
+  // statistics passes of at most 64 samples per unit (the adaptive pilot) trace each round once
+  const bool one = U.stats && !U.omm_counts && !U.unit_counts && U.spp <= 64;
+  if (U.stats && U.samples && one)
+    hipLaunchKernelGGL((k_render_units<true, true, kTrav, true>), grid, block, lds, stream, v, U);
+  else if (U.stats && one)
+    hipLaunchKernelGGL((k_render_units<true, false, kTrav, true>), grid, block, lds, stream, v, U);
+  else if (U.stats && U.samples)
     hipLaunchKernelGGL((k_render_units<true, true, kTrav>), grid, block, lds, stream, v, U);
   else if (U.stats)
     hipLaunchKernelGGL((k_render_units<true, false, kTrav>), grid, block, lds, stream, v, U);
@@ -1274,6 +1283,9 @@ static void launch_chunks(const UnitLaunch& U, const SceneView& v, int blocks, s
 
 template <int kTrav>
 static const void* units_kernel_variant(const UnitLaunch& U) {
+  const bool one = U.stats && !U.omm_counts && !U.unit_counts && U.spp <= 64;
+  if (U.stats && U.samples && one) return reinterpret_cast<const void*>(&k_render_units<true, true, kTrav, true>);
+  if (U.stats && one) return reinterpret_cast<const void*>(&k_render_units<true, false, kTrav, true>);
   if (U.stats && U.samples) return reinterpret_cast<const void*>(&k_render_units<true, true, kTrav>);
   if (U.stats) return reinterpret_cast<const void*>(&k_render_units<true, false, kTrav>);
   if (U.samples) return reinterpret_cast<const void*>(&k_render_units<false, true, kTrav>);
