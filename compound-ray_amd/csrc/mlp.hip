@@ -32,15 +32,17 @@ constexpr int P_W4 = P_W3N + 64 * 64;    // [64]
 constexpr int P_TOTAL = P_W4 + 64;       // 17472 (multiple of 8)
 static_assert(P_TOTAL % 8 == 0, "packed size must allow 16-byte copies");
 
+// FP32 -> BF16, round to nearest even (a NaN stays a quiet NaN).  On the GPU
+// this is gfx950's one-instruction conversion (v_cvt_pk_bf16_f32).
 __host__ __device__ inline uint16_t f2bf(float f) {
-  uint32_t u;
 #if defined(__HIP_DEVICE_COMPILE__)
-  u = __float_as_uint(f);
+  return __builtin_bit_cast(uint16_t, (__bf16)f);
 #else
+  uint32_t u;
   memcpy(&u, &f, 4);
-#endif
   if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
   return (uint16_t)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+#endif
 }
 __device__ inline float bf2f(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
 __device__ inline float bf2f_s(short b) { return bf2f((uint16_t)b); }
