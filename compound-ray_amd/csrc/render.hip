@@ -1252,12 +1252,17 @@ template <int kTrav>
 static void launch_units_variant(const UnitLaunch& U, const SceneView& v, int blocks, size_t lds,
                                  hipStream_t stream) {
   const dim3 grid(blocks), block(kTraceThreads);
-  // statistics passes of at most 64 samples per unit (the adaptive pilot) trace each round once
-  const bool one = U.stats && !U.omm_counts && !U.unit_counts && U.spp <= 64;
+  // a uniform count of at most 64 samples per unit (the adaptive pilot, uniform
+  // batches of few samples): every round traces once
+  const bool one = !U.omm_counts && !U.unit_counts && U.spp <= 64;
   if (U.stats && U.samples && one)
     hipLaunchKernelGGL((k_render_units<true, true, kTrav, true>), grid, block, lds, stream, v, U);
   else if (U.stats && one)
     hipLaunchKernelGGL((k_render_units<true, false, kTrav, true>), grid, block, lds, stream, v, U);
+  else if (U.samples && one)
+    hipLaunchKernelGGL((k_render_units<false, true, kTrav, true>), grid, block, lds, stream, v, U);
+  else if (one)
+    hipLaunchKernelGGL((k_render_units<false, false, kTrav, true>), grid, block, lds, stream, v, U);
   else if (U.stats && U.samples)
     hipLaunchKernelGGL((k_render_units<true, true, kTrav>), grid, block, lds, stream, v, U);
   else if (U.stats)
@@ -1283,9 +1288,11 @@ static void launch_chunks(const UnitLaunch& U, const SceneView& v, int blocks, s
 
 template <int kTrav>
 static const void* units_kernel_variant(const UnitLaunch& U) {
-  const bool one = U.stats && !U.omm_counts && !U.unit_counts && U.spp <= 64;
+  const bool one = !U.omm_counts && !U.unit_counts && U.spp <= 64;
   if (U.stats && U.samples && one) return reinterpret_cast<const void*>(&k_render_units<true, true, kTrav, true>);
   if (U.stats && one) return reinterpret_cast<const void*>(&k_render_units<true, false, kTrav, true>);
+  if (U.samples && one) return reinterpret_cast<const void*>(&k_render_units<false, true, kTrav, true>);
+  if (one) return reinterpret_cast<const void*>(&k_render_units<false, false, kTrav, true>);
   if (U.stats && U.samples) return reinterpret_cast<const void*>(&k_render_units<true, true, kTrav>);
   if (U.stats) return reinterpret_cast<const void*>(&k_render_units<true, false, kTrav>);
   if (U.samples) return reinterpret_cast<const void*>(&k_render_units<false, true, kTrav>);
